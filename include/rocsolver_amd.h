@@ -1,0 +1,231 @@
+/* rocsolver_amd.h -- C ABI of the MI355X (gfx950) batched LU engine.
+ *
+ * Drop-in for the batched dense LU factor-and-solve path of rocsolver.h:
+ * every function below has the signature, argument meaning, argument-check
+ * order, status codes and in/out semantics of the rocSOLVER function of the
+ * same name.  Citations are to the reference source tree
+ * (library/include/rocsolver/rocsolver-functions.h and the implementation
+ * files named).
+ *
+ * Conventions shared by all functions (as in the reference):
+ *  - matrices are column-major, element (i,j) at A[i + j*lda];
+ *  - all matrix, pivot and info arrays are caller-owned DEVICE memory;
+ *  - _strided_batched: matrix b at A + b*strideA; _batched: A is a device
+ *    array of bc device pointers; ipiv is strided by strideP in both forms;
+ *  - ipiv is 1-based; info[b] = first zero pivot (1-based) or 0, and a zero
+ *    pivot is not an error status;
+ *  - work is queued on rocblas_get_stream(handle), asynchronous to the host,
+ *    with no host synchronisation inside a call.
+ *
+ * Extensions with no reference counterpart are prefixed rocsolver_amd_.
+ */
+#ifndef ROCSOLVER_AMD_H
+#define ROCSOLVER_AMD_H
+
+#include <rocblas/rocblas.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ GETRF
+ * LU factorisation with partial pivoting, A = P L U.
+ * Reference: rocsolver-functions.h:5139/5147 (getrf), 5228/5238 (_batched),
+ * 5329/5340 (_strided_batched); implementation
+ * library/src/lapack/roclapack_getrf{,_batched,_strided_batched}.cpp. */
+rocblas_status rocsolver_sgetrf(rocblas_handle handle, const rocblas_int m, const rocblas_int n,
+                                float* A, const rocblas_int lda, rocblas_int* ipiv,
+                                rocblas_int* info);
+rocblas_status rocsolver_dgetrf(rocblas_handle handle, const rocblas_int m, const rocblas_int n,
+                                double* A, const rocblas_int lda, rocblas_int* ipiv,
+                                rocblas_int* info);
+rocblas_status rocsolver_sgetrf_batched(rocblas_handle handle, const rocblas_int m,
+                                        const rocblas_int n, float* const A[],
+                                        const rocblas_int lda, rocblas_int* ipiv,
+                                        const rocblas_stride strideP, rocblas_int* info,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrf_batched(rocblas_handle handle, const rocblas_int m,
+                                        const rocblas_int n, double* const A[],
+                                        const rocblas_int lda, rocblas_int* ipiv,
+                                        const rocblas_stride strideP, rocblas_int* info,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_sgetrf_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                const rocblas_int n, float* A,
+                                                const rocblas_int lda,
+                                                const rocblas_stride strideA, rocblas_int* ipiv,
+                                                const rocblas_stride strideP, rocblas_int* info,
+                                                const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrf_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                const rocblas_int n, double* A,
+                                                const rocblas_int lda,
+                                                const rocblas_stride strideA, rocblas_int* ipiv,
+                                                const rocblas_stride strideP, rocblas_int* info,
+                                                const rocblas_int batch_count);
+
+/* Non-pivoting LU, A = L U.  Reference: rocsolver-functions.h:4609/4616
+ * (getrf_npvt), 4687/4695 (_batched), 4773/4782 (_strided_batched);
+ * roclapack_getrf_strided_batched.cpp:154-208. */
+rocblas_status rocsolver_sgetrf_npvt(rocblas_handle handle, const rocblas_int m,
+                                     const rocblas_int n, float* A, const rocblas_int lda,
+                                     rocblas_int* info);
+rocblas_status rocsolver_dgetrf_npvt(rocblas_handle handle, const rocblas_int m,
+                                     const rocblas_int n, double* A, const rocblas_int lda,
+                                     rocblas_int* info);
+rocblas_status rocsolver_sgetrf_npvt_batched(rocblas_handle handle, const rocblas_int m,
+                                             const rocblas_int n, float* const A[],
+                                             const rocblas_int lda, rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrf_npvt_batched(rocblas_handle handle, const rocblas_int m,
+                                             const rocblas_int n, double* const A[],
+                                             const rocblas_int lda, rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_sgetrf_npvt_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                     const rocblas_int n, float* A,
+                                                     const rocblas_int lda,
+                                                     const rocblas_stride strideA,
+                                                     rocblas_int* info,
+                                                     const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrf_npvt_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                     const rocblas_int n, double* A,
+                                                     const rocblas_int lda,
+                                                     const rocblas_stride strideA,
+                                                     rocblas_int* info,
+                                                     const rocblas_int batch_count);
+
+/* Unblocked LU (same results contract as getrf).  Reference:
+ * rocsolver-functions.h:4857/4865 (getf2), 4946/4956 (_batched), 5047/5058
+ * (_strided_batched); library/src/lapack/roclapack_getf2{,_batched,_strided_batched}.cpp. */
+rocblas_status rocsolver_sgetf2(rocblas_handle handle, const rocblas_int m, const rocblas_int n,
+                                float* A, const rocblas_int lda, rocblas_int* ipiv,
+                                rocblas_int* info);
+rocblas_status rocsolver_dgetf2(rocblas_handle handle, const rocblas_int m, const rocblas_int n,
+                                double* A, const rocblas_int lda, rocblas_int* ipiv,
+                                rocblas_int* info);
+rocblas_status rocsolver_sgetf2_batched(rocblas_handle handle, const rocblas_int m,
+                                        const rocblas_int n, float* const A[],
+                                        const rocblas_int lda, rocblas_int* ipiv,
+                                        const rocblas_stride strideP, rocblas_int* info,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_dgetf2_batched(rocblas_handle handle, const rocblas_int m,
+                                        const rocblas_int n, double* const A[],
+                                        const rocblas_int lda, rocblas_int* ipiv,
+                                        const rocblas_stride strideP, rocblas_int* info,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_sgetf2_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                const rocblas_int n, float* A,
+                                                const rocblas_int lda,
+                                                const rocblas_stride strideA, rocblas_int* ipiv,
+                                                const rocblas_stride strideP, rocblas_int* info,
+                                                const rocblas_int batch_count);
+rocblas_status rocsolver_dgetf2_strided_batched(rocblas_handle handle, const rocblas_int m,
+                                                const rocblas_int n, double* A,
+                                                const rocblas_int lda,
+                                                const rocblas_stride strideA, rocblas_int* ipiv,
+                                                const rocblas_stride strideP, rocblas_int* info,
+                                                const rocblas_int batch_count);
+
+/* ------------------------------------------------------------------ GETRS
+ * Solve op(A) X = B with the LU from getrf; B overwritten by X.
+ * Reference: rocsolver-functions.h:8562/8572 (getrs), 8657/8669 (_batched),
+ * 8768/8782 (_strided_batched); library/src/lapack/roclapack_getrs.hpp. */
+rocblas_status rocsolver_sgetrs(rocblas_handle handle, const rocblas_operation trans,
+                                const rocblas_int n, const rocblas_int nrhs, float* A,
+                                const rocblas_int lda, const rocblas_int* ipiv, float* B,
+                                const rocblas_int ldb);
+rocblas_status rocsolver_dgetrs(rocblas_handle handle, const rocblas_operation trans,
+                                const rocblas_int n, const rocblas_int nrhs, double* A,
+                                const rocblas_int lda, const rocblas_int* ipiv, double* B,
+                                const rocblas_int ldb);
+rocblas_status rocsolver_sgetrs_batched(rocblas_handle handle, const rocblas_operation trans,
+                                        const rocblas_int n, const rocblas_int nrhs,
+                                        float* const A[], const rocblas_int lda,
+                                        const rocblas_int* ipiv, const rocblas_stride strideP,
+                                        float* const B[], const rocblas_int ldb,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrs_batched(rocblas_handle handle, const rocblas_operation trans,
+                                        const rocblas_int n, const rocblas_int nrhs,
+                                        double* const A[], const rocblas_int lda,
+                                        const rocblas_int* ipiv, const rocblas_stride strideP,
+                                        double* const B[], const rocblas_int ldb,
+                                        const rocblas_int batch_count);
+rocblas_status rocsolver_sgetrs_strided_batched(rocblas_handle handle,
+                                                const rocblas_operation trans,
+                                                const rocblas_int n, const rocblas_int nrhs,
+                                                float* A, const rocblas_int lda,
+                                                const rocblas_stride strideA,
+                                                const rocblas_int* ipiv,
+                                                const rocblas_stride strideP, float* B,
+                                                const rocblas_int ldb,
+                                                const rocblas_stride strideB,
+                                                const rocblas_int batch_count);
+rocblas_status rocsolver_dgetrs_strided_batched(rocblas_handle handle,
+                                                const rocblas_operation trans,
+                                                const rocblas_int n, const rocblas_int nrhs,
+                                                double* A, const rocblas_int lda,
+                                                const rocblas_stride strideA,
+                                                const rocblas_int* ipiv,
+                                                const rocblas_stride strideP, double* B,
+                                                const rocblas_int ldb,
+                                                const rocblas_stride strideB,
+                                                const rocblas_int batch_count);
+
+/* ------------------------------------------------------------------- GESV
+ * Solve A X = B (getrf + getrs); A overwritten by its LU, B by X, except that
+ * B is left unchanged for systems with info != 0.
+ * Reference: rocsolver-functions.h:8872/8882 (gesv), 8968/8980 (_batched),
+ * 9081/9095 (_strided_batched); library/src/lapack/roclapack_gesv.hpp. */
+rocblas_status rocsolver_sgesv(rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs,
+                               float* A, const rocblas_int lda, rocblas_int* ipiv, float* B,
+                               const rocblas_int ldb, rocblas_int* info);
+rocblas_status rocsolver_dgesv(rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs,
+                               double* A, const rocblas_int lda, rocblas_int* ipiv, double* B,
+                               const rocblas_int ldb, rocblas_int* info);
+rocblas_status rocsolver_sgesv_batched(rocblas_handle handle, const rocblas_int n,
+                                       const rocblas_int nrhs, float* const A[],
+                                       const rocblas_int lda, rocblas_int* ipiv,
+                                       const rocblas_stride strideP, float* const B[],
+                                       const rocblas_int ldb, rocblas_int* info,
+                                       const rocblas_int batch_count);
+rocblas_status rocsolver_dgesv_batched(rocblas_handle handle, const rocblas_int n,
+                                       const rocblas_int nrhs, double* const A[],
+                                       const rocblas_int lda, rocblas_int* ipiv,
+                                       const rocblas_stride strideP, double* const B[],
+                                       const rocblas_int ldb, rocblas_int* info,
+                                       const rocblas_int batch_count);
+rocblas_status rocsolver_sgesv_strided_batched(rocblas_handle handle, const rocblas_int n,
+                                               const rocblas_int nrhs, float* A,
+                                               const rocblas_int lda,
+                                               const rocblas_stride strideA, rocblas_int* ipiv,
+                                               const rocblas_stride strideP, float* B,
+                                               const rocblas_int ldb,
+                                               const rocblas_stride strideB, rocblas_int* info,
+                                               const rocblas_int batch_count);
+rocblas_status rocsolver_dgesv_strided_batched(rocblas_handle handle, const rocblas_int n,
+                                               const rocblas_int nrhs, double* A,
+                                               const rocblas_int lda,
+                                               const rocblas_stride strideA, rocblas_int* ipiv,
+                                               const rocblas_stride strideP, double* B,
+                                               const rocblas_int ldb,
+                                               const rocblas_stride strideB, rocblas_int* info,
+                                               const rocblas_int batch_count);
+
+/* ------------------------------------------------------------------ LASWP
+ * Row interchanges driven by ipiv[k1-1 .. k2-1] (incx = +-1).
+ * Reference: rocsolver-functions.h:198/207;
+ * library/src/auxiliary/rocauxiliary_laswp.{cpp,hpp}. */
+rocblas_status rocsolver_slaswp(rocblas_handle handle, const rocblas_int n, float* A,
+                                const rocblas_int lda, const rocblas_int k1, const rocblas_int k2,
+                                const rocblas_int* ipiv, const rocblas_int incx);
+rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, double* A,
+                                const rocblas_int lda, const rocblas_int k1, const rocblas_int k2,
+                                const rocblas_int* ipiv, const rocblas_int incx);
+
+/* ---------------------------------------------------------------- version
+ * Reference: rocsolver-functions.h:34 and :45; library/src/common/buildinfo.cpp. */
+rocblas_status rocsolver_get_version_string(char* buf, size_t len);
+rocblas_status rocsolver_get_version_string_size(size_t* len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ROCSOLVER_AMD_H */

@@ -1,0 +1,240 @@
+"""rocsolver_amd -- MI355X-native batched LU factor-and-solve engine.
+
+The product is the C-ABI shared library ``librocsolver_amd.so`` (declared in
+``include/rocsolver_amd.h``): rocsolver_{s,d}{getrf,getf2,getrf_npvt,getrs,gesv}
+in plain, ``_batched`` and ``_strided_batched`` forms, and rocsolver_{s,d}laswp,
+with the rocsolver.h signatures and semantics.  This module is a thin Python
+host mirror of that boundary: it loads the library (loudly failing when it is
+missing -- there is no fallback path), wraps a ``rocblas_handle`` and offers
+torch-tensor convenience calls with the reference's argument meaning.
+
+Batch layout used by the helpers: a tensor of shape (bc, ncols, ld), C
+contiguous, i.e. matrix b is column-major at ``base + b*ncols*ld`` -- exactly
+the strided-batched device layout with strideA = ncols*ld.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librocsolver_amd.so")
+ROCBLAS_PATH = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "librocblas.so")
+
+# rocblas_status values (rocblas-types.h)
+STATUS = {
+    0: "success", 1: "invalid_handle", 2: "not_implemented", 3: "invalid_pointer",
+    4: "invalid_size", 5: "memory_error", 6: "internal_error", 7: "perf_degraded",
+    8: "size_query_mismatch", 9: "size_increased", 10: "size_unchanged", 11: "invalid_value",
+    12: "continue",
+}
+OP_NONE, OP_TRANSPOSE, OP_CONJ_TRANSPOSE = 111, 112, 113
+
+_lib = None
+_rocblas = None
+
+
+class RocsolverError(RuntimeError):
+    def __init__(self, name: str, status: int):
+        super().__init__(f"{name} returned rocblas_status {status} ({STATUS.get(status, '?')})")
+        self.status = status
+
+
+def lib() -> ctypes.CDLL:
+    """The engine's shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `python -m rocsolver_amd.build` "
+                          "(there is no fallback implementation)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def rocblas() -> ctypes.CDLL:
+    global _rocblas
+    if _rocblas is None:
+        _rocblas = ctypes.CDLL(ROCBLAS_PATH)
+        _rocblas.rocblas_create_handle.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _rocblas.rocblas_destroy_handle.argtypes = [ctypes.c_void_p]
+        _rocblas.rocblas_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _rocblas.rocblas_start_device_memory_size_query.argtypes = [ctypes.c_void_p]
+        _rocblas.rocblas_stop_device_memory_size_query.argtypes = [
+            ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)]
+    return _rocblas
+
+
+_I, _P, _S = ctypes.c_int, ctypes.c_void_p, ctypes.c_int64
+
+# argument lists, in declaration order (include/rocsolver_amd.h)
+SIGNATURES = {
+    "getrf": [_P, _I, _I, _P, _I, _P, _P],
+    "getrf_batched": [_P, _I, _I, _P, _I, _P, _S, _P, _I],
+    "getrf_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I],
+    "getf2": [_P, _I, _I, _P, _I, _P, _P],
+    "getf2_batched": [_P, _I, _I, _P, _I, _P, _S, _P, _I],
+    "getf2_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I],
+    "getrf_npvt": [_P, _I, _I, _P, _I, _P],
+    "getrf_npvt_batched": [_P, _I, _I, _P, _I, _P, _I],
+    "getrf_npvt_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _I],
+    "getrs": [_P, _I, _I, _I, _P, _I, _P, _P, _I],
+    "getrs_batched": [_P, _I, _I, _I, _P, _I, _P, _S, _P, _I, _I],
+    "getrs_strided_batched": [_P, _I, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S, _I],
+    "gesv": [_P, _I, _I, _P, _I, _P, _P, _I, _P],
+    "gesv_batched": [_P, _I, _I, _P, _I, _P, _S, _P, _I, _P, _I],
+    "gesv_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S, _P, _I],
+    "laswp": [_P, _I, _P, _I, _I, _I, _P, _I],
+}
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    for base, args in SIGNATURES.items():
+        for pre in ("s", "d"):
+            f = getattr(L, f"rocsolver_{pre}{base}")
+            f.argtypes = args
+            f.restype = ctypes.c_int
+    L.rocsolver_get_version_string.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+    L.rocsolver_get_version_string_size.argtypes = [ctypes.POINTER(ctypes.c_size_t)]
+
+
+def exported_symbols() -> list[str]:
+    out = [f"rocsolver_{p}{b}" for b in SIGNATURES for p in ("s", "d")]
+    return out + ["rocsolver_get_version_string", "rocsolver_get_version_string_size"]
+
+
+def version() -> str:
+    n = ctypes.c_size_t()
+    lib().rocsolver_get_version_string_size(ctypes.byref(n))
+    buf = ctypes.create_string_buffer(n.value)
+    lib().rocsolver_get_version_string(buf, n.value)
+    return buf.value.decode()
+
+
+def call(name: str, *args) -> int:
+    """Call rocsolver_<name> with raw arguments; returns the status."""
+    return getattr(lib(), name)(*args)
+
+
+def check(name: str, *args) -> None:
+    st = call(name, *args)
+    if st != 0:
+        raise RocsolverError(name, st)
+
+
+class Handle:
+    """A rocblas_handle bound to torch's current HIP stream."""
+
+    def __init__(self, stream=None):
+        import torch
+        h = ctypes.c_void_p()
+        st = rocblas().rocblas_create_handle(ctypes.byref(h))
+        if st != 0:
+            raise RocsolverError("rocblas_create_handle", st)
+        self.value = h.value
+        s = stream if stream is not None else torch.cuda.current_stream()
+        rocblas().rocblas_set_stream(self.value, ctypes.c_void_p(s.cuda_stream))
+
+    def set_stream(self, stream) -> None:
+        rocblas().rocblas_set_stream(self.value, ctypes.c_void_p(stream.cuda_stream))
+
+    def close(self) -> None:
+        if self.value:
+            rocblas().rocblas_destroy_handle(self.value)
+            self.value = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _prefix(t) -> str:
+    import torch
+    if t.dtype == torch.float64:
+        return "d"
+    if t.dtype == torch.float32:
+        return "s"
+    raise TypeError(f"float32/float64 only, got {t.dtype}")
+
+
+def pointer_array(t):
+    """Device array of per-matrix pointers for the _batched forms."""
+    import torch
+    bc = t.shape[0]
+    stride = t.stride(0) * t.element_size()
+    base = t.data_ptr()
+    ptrs = torch.arange(bc, dtype=torch.int64) * stride + base
+    return ptrs.to(t.device)
+
+
+# ------------------------------------------------------------ torch helpers
+def getrf(handle: Handle, A, m: int | None = None, pivot: bool = True, batched: bool = False):
+    """LU of every matrix of A (bc, n, lda) in place -> (ipiv[bc, min(m,n)], info[bc])."""
+    import torch
+    bc, n, lda = A.shape
+    m = lda if m is None else m
+    k = min(m, n)
+    dev = A.device
+    ipiv = torch.zeros((bc, max(k, 1)), dtype=torch.int32, device=dev)
+    info = torch.zeros(bc, dtype=torch.int32, device=dev)
+    p = _prefix(A)
+    if batched:
+        ptrs = pointer_array(A)
+        if pivot:
+            check(f"rocsolver_{p}getrf_batched", handle.value, m, n, ptrs.data_ptr(), lda,
+                  ipiv.data_ptr(), ipiv.shape[1], info.data_ptr(), bc)
+        else:
+            check(f"rocsolver_{p}getrf_npvt_batched", handle.value, m, n, ptrs.data_ptr(), lda,
+                  info.data_ptr(), bc)
+    else:
+        if pivot:
+            check(f"rocsolver_{p}getrf_strided_batched", handle.value, m, n, A.data_ptr(), lda,
+                  n * lda, ipiv.data_ptr(), ipiv.shape[1], info.data_ptr(), bc)
+        else:
+            check(f"rocsolver_{p}getrf_npvt_strided_batched", handle.value, m, n, A.data_ptr(),
+                  lda, n * lda, info.data_ptr(), bc)
+    return ipiv[:, :k], info
+
+
+def getrs(handle: Handle, LU, ipiv, B, trans: str = "N", batched: bool = False):
+    """Solve op(A) X = B in place for B (bc, nrhs, ldb); LU (bc, n, lda), ipiv (bc, >=n)."""
+    bc, n, lda = LU.shape
+    _, nrhs, ldb = B.shape
+    op = {"N": OP_NONE, "T": OP_TRANSPOSE, "C": OP_CONJ_TRANSPOSE}[trans.upper()]
+    p = _prefix(LU)
+    sp = ipiv.stride(0)
+    if batched:
+        pa, pb = pointer_array(LU), pointer_array(B)
+        check(f"rocsolver_{p}getrs_batched", handle.value, op, n, nrhs, pa.data_ptr(), lda,
+              ipiv.data_ptr(), sp, pb.data_ptr(), ldb, bc)
+    else:
+        check(f"rocsolver_{p}getrs_strided_batched", handle.value, op, n, nrhs, LU.data_ptr(),
+              lda, n * lda, ipiv.data_ptr(), sp, B.data_ptr(), ldb, nrhs * ldb, bc)
+
+
+def gesv(handle: Handle, A, B, batched: bool = False):
+    """Solve A X = B in place (A -> LU, B -> X) -> (ipiv, info)."""
+    import torch
+    bc, n, lda = A.shape
+    _, nrhs, ldb = B.shape
+    dev = A.device
+    ipiv = torch.zeros((bc, max(n, 1)), dtype=torch.int32, device=dev)
+    info = torch.zeros(bc, dtype=torch.int32, device=dev)
+    p = _prefix(A)
+    if batched:
+        pa, pb = pointer_array(A), pointer_array(B)
+        check(f"rocsolver_{p}gesv_batched", handle.value, n, nrhs, pa.data_ptr(), lda,
+              ipiv.data_ptr(), ipiv.shape[1], pb.data_ptr(), ldb, info.data_ptr(), bc)
+    else:
+        check(f"rocsolver_{p}gesv_strided_batched", handle.value, n, nrhs, A.data_ptr(), lda,
+              n * lda, ipiv.data_ptr(), ipiv.shape[1], B.data_ptr(), ldb, nrhs * ldb,
+              info.data_ptr(), bc)
+    return ipiv[:, :n], info

@@ -1,0 +1,309 @@
+// api.cpp -- extern "C" entry points (the drop-in boundary).
+//
+// Each *_impl reproduces the reference impl's control flow:
+//   null handle -> invalid_handle;  argCheck in the reference order
+//   (values, then sizes, then -- unless in device-memory size-query mode --
+//   pointers);  size query -> report the workspace (none is needed here);
+//   quick returns;  then the work, queued on the handle's stream.
+// References: roclapack_getrf_strided_batched.cpp:7-84 (getrf impl),
+// roclapack_getf2.hpp:488-517 (getf2/getrf argCheck), roclapack_getrs.hpp:17-49
+// (getrs argCheck) and :99-107 (quick return), roclapack_gesv.hpp:17-46
+// (gesv argCheck) and :140-152 (info reset and quick return),
+// rocauxiliary_laswp.hpp:66-94 and rocauxiliary_laswp.cpp:7-42 (laswp).
+#include "../../include/rocsolver_amd.h"
+#include "internal.hpp"
+
+#include <rocblas/internal/rocblas_device_malloc.hpp>
+
+#include <cstring>
+
+using namespace rsa;
+
+namespace
+{
+
+// the work needs no device workspace: a size query records zero bytes
+rocblas_status report_no_workspace(rocblas_handle handle)
+{
+    return rocblas_set_optimal_device_memory_size(handle, size_t(0));
+}
+
+template <typename T>
+rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool hasA, int lda,
+                          int* ipiv, int64_t strideP, int* info, int bc, bool pivot)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(m < 0 || n < 0 || lda < m || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    const int64_t mn = int64_t(m) * n;
+    if(!query && ((mn && !hasA) || (mn && pivot && !ipiv) || (bc && !info)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(bc == 0)
+        return rocblas_status_success;
+    if(m == 0 || n == 0)
+        return reset_info(stream, info, bc);
+    if(m <= SMALL_MAX && n <= SMALL_MAX)
+        return getrf_small<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
+    return getrf_blocked<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
+}
+
+rocblas_status check_trans(rocblas_operation trans, bool* transposed)
+{
+    if(trans != rocblas_operation_none && trans != rocblas_operation_transpose
+       && trans != rocblas_operation_conjugate_transpose)
+        return rocblas_status_invalid_value;
+    *transposed = trans != rocblas_operation_none; // real data: C == T
+    return rocblas_status_continue;
+}
+
+template <typename T>
+rocblas_status getrs_impl(rocblas_handle handle, rocblas_operation trans, int n, int nrhs,
+                          Batch<T> A, bool hasA, int lda, const int* ipiv, int64_t strideP,
+                          Batch<T> B, bool hasB, int ldb, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    bool tr;
+    rocblas_status st = check_trans(trans, &tr);
+    if(st != rocblas_status_continue)
+        return st;
+    if(n < 0 || nrhs < 0 || lda < n || ldb < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && ((n && !hasA) || (n && !ipiv) || (int64_t(nrhs) * n && !hasB)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+
+    if(n == 0 || nrhs == 0 || bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(n <= SMALL_MAX)
+        return getrs_small<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true);
+    return getrs_blocked<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
+                            nullptr);
+}
+
+template <typename T>
+rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, bool hasA, int lda,
+                         int* ipiv, int64_t strideP, Batch<T> B, bool hasB, int ldb, int* info,
+                         int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || nrhs < 0 || lda < n || ldb < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query
+       && ((n && !hasA) || (n && !ipiv) || (int64_t(nrhs) * n && !hasB) || (bc && !info)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(n == 0 || nrhs == 0)
+        return reset_info(stream, info, bc);
+    if(n <= SMALL_MAX)
+        return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
+    rocblas_status st = getrf_blocked<T>(stream, n, n, A, lda, ipiv, strideP, info, bc, true);
+    if(st != rocblas_status_success)
+        return st;
+    return getrs_blocked<T>(stream, false, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
+                            info);
+}
+
+template <typename T>
+rocblas_status laswp_impl(rocblas_handle handle, int n, T* A, int lda, int k1, int k2,
+                          const int* ipiv, int incx)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || lda < 1 || !incx || k1 < 1 || k2 < 1 || k2 < k1)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && ((n && !A) || !ipiv))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return rocblas_status_size_unchanged;
+    if(n == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    // pivots are read as ipiv[k1 + (i-k1)*|incx| - 1], as the reference does
+    return laswp<T>(stream, n, strided(A, 0), lda, k1, k2, ipiv, 0, incx, 1, nullptr);
+}
+
+} // namespace
+
+#define RSA_GETRF_FAMILY(P, T)                                                                  \
+    rocblas_status rocsolver_##P##getrf(rocblas_handle handle, const rocblas_int m,             \
+                                        const rocblas_int n, T* A, const rocblas_int lda,       \
+                                        rocblas_int* ipiv, rocblas_int* info)                   \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, ipiv, 0, info, 1, \
+                             true);                                                             \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrf_batched(rocblas_handle handle, const rocblas_int m,     \
+                                                const rocblas_int n, T* const A[],              \
+                                                const rocblas_int lda, rocblas_int* ipiv,       \
+                                                const rocblas_stride strideP,                   \
+                                                rocblas_int* info, const rocblas_int bc)        \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,     \
+                             info, bc, true);                                                   \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrf_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_int m, const rocblas_int n, T* A,                  \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,                 \
+        const rocblas_stride strideP, rocblas_int* info, const rocblas_int bc)                  \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, ipiv,        \
+                             strideP, info, bc, true);                                          \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getf2(rocblas_handle handle, const rocblas_int m,             \
+                                        const rocblas_int n, T* A, const rocblas_int lda,       \
+                                        rocblas_int* ipiv, rocblas_int* info)                   \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, ipiv, 0, info, 1, \
+                             true);                                                             \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getf2_batched(rocblas_handle handle, const rocblas_int m,     \
+                                                const rocblas_int n, T* const A[],              \
+                                                const rocblas_int lda, rocblas_int* ipiv,       \
+                                                const rocblas_stride strideP,                   \
+                                                rocblas_int* info, const rocblas_int bc)        \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,     \
+                             info, bc, true);                                                   \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getf2_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_int m, const rocblas_int n, T* A,                  \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,                 \
+        const rocblas_stride strideP, rocblas_int* info, const rocblas_int bc)                  \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, ipiv,        \
+                             strideP, info, bc, true);                                          \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrf_npvt(rocblas_handle handle, const rocblas_int m,        \
+                                             const rocblas_int n, T* A, const rocblas_int lda,  \
+                                             rocblas_int* info)                                 \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, nullptr, 0, info,  \
+                             1, false);                                                         \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrf_npvt_batched(rocblas_handle handle, const rocblas_int m, \
+                                                     const rocblas_int n, T* const A[],         \
+                                                     const rocblas_int lda, rocblas_int* info,  \
+                                                     const rocblas_int bc)                      \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, nullptr, 0, info,  \
+                             bc, false);                                                        \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrf_npvt_strided_batched(                                   \
+        rocblas_handle handle, const rocblas_int m, const rocblas_int n, T* A,                  \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
+        const rocblas_int bc)                                                                   \
+    {                                                                                           \
+        return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,  \
+                             info, bc, false);                                                  \
+    }
+
+#define RSA_GETRS_GESV_FAMILY(P, T)                                                             \
+    rocblas_status rocsolver_##P##getrs(rocblas_handle handle, const rocblas_operation trans,   \
+                                        const rocblas_int n, const rocblas_int nrhs, T* A,      \
+                                        const rocblas_int lda, const rocblas_int* ipiv, T* B,   \
+                                        const rocblas_int ldb)                                  \
+    {                                                                                           \
+        return getrs_impl<T>(handle, trans, n, nrhs, strided(A, 0), A != nullptr, lda, ipiv, 0, \
+                             strided(B, 0), B != nullptr, ldb, 1);                              \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrs_batched(                                                \
+        rocblas_handle handle, const rocblas_operation trans, const rocblas_int n,              \
+        const rocblas_int nrhs, T* const A[], const rocblas_int lda, const rocblas_int* ipiv,    \
+        const rocblas_stride strideP, T* const B[], const rocblas_int ldb, const rocblas_int bc) \
+    {                                                                                           \
+        return getrs_impl<T>(handle, trans, n, nrhs, arrayed<T>(A), A != nullptr, lda, ipiv,    \
+                             strideP, arrayed<T>(B), B != nullptr, ldb, bc);                    \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getrs_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_operation trans, const rocblas_int n,              \
+        const rocblas_int nrhs, T* A, const rocblas_int lda, const rocblas_stride strideA,      \
+        const rocblas_int* ipiv, const rocblas_stride strideP, T* B, const rocblas_int ldb,     \
+        const rocblas_stride strideB, const rocblas_int bc)                                     \
+    {                                                                                           \
+        return getrs_impl<T>(handle, trans, n, nrhs, strided(A, strideA), A != nullptr, lda,    \
+                             ipiv, strideP, strided(B, strideB), B != nullptr, ldb, bc);        \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##gesv(rocblas_handle handle, const rocblas_int n,              \
+                                       const rocblas_int nrhs, T* A, const rocblas_int lda,     \
+                                       rocblas_int* ipiv, T* B, const rocblas_int ldb,          \
+                                       rocblas_int* info)                                       \
+    {                                                                                           \
+        return gesv_impl<T>(handle, n, nrhs, strided(A, 0), A != nullptr, lda, ipiv, 0,         \
+                            strided(B, 0), B != nullptr, ldb, info, 1);                         \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##gesv_batched(                                                 \
+        rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, T* const A[],       \
+        const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP, T* const B[],   \
+        const rocblas_int ldb, rocblas_int* info, const rocblas_int bc)                         \
+    {                                                                                           \
+        return gesv_impl<T>(handle, n, nrhs, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,   \
+                            arrayed<T>(B), B != nullptr, ldb, info, bc);                        \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##gesv_strided_batched(                                         \
+        rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, T* A,               \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,                 \
+        const rocblas_stride strideP, T* B, const rocblas_int ldb,                              \
+        const rocblas_stride strideB, rocblas_int* info, const rocblas_int bc)                  \
+    {                                                                                           \
+        return gesv_impl<T>(handle, n, nrhs, strided(A, strideA), A != nullptr, lda, ipiv,      \
+                            strideP, strided(B, strideB), B != nullptr, ldb, info, bc);         \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##laswp(rocblas_handle handle, const rocblas_int n, T* A,       \
+                                        const rocblas_int lda, const rocblas_int k1,            \
+                                        const rocblas_int k2, const rocblas_int* ipiv,          \
+                                        const rocblas_int incx)                                 \
+    {                                                                                           \
+        return laswp_impl<T>(handle, n, A, lda, k1, k2, ipiv, incx);                            \
+    }
+
+extern "C" {
+
+RSA_GETRF_FAMILY(s, float)
+RSA_GETRF_FAMILY(d, double)
+RSA_GETRS_GESV_FAMILY(s, float)
+RSA_GETRS_GESV_FAMILY(d, double)
+
+#define RSA_VERSION_STRING "3.23.0.gfx950"
+
+rocblas_status rocsolver_get_version_string(char* buf, size_t len)
+{
+    static constexpr char v[] = RSA_VERSION_STRING;
+    if(!buf)
+        return rocblas_status_invalid_pointer;
+    if(len < sizeof(v))
+        return rocblas_status_invalid_size;
+    std::memcpy(buf, v, sizeof(v));
+    return rocblas_status_success;
+}
+
+rocblas_status rocsolver_get_version_string_size(size_t* len)
+{
+    if(!len)
+        return rocblas_status_invalid_pointer;
+    *len = std::strlen(RSA_VERSION_STRING) + 1;
+    return rocblas_status_success;
+}
+
+} // extern "C"

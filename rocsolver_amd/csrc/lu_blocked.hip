@@ -1,0 +1,621 @@
+// lu_blocked.hip -- right-looking blocked LU and blocked triangular solves for
+// any m, n (the path for matrices beyond the register-resident kernels).
+//
+// getrf, per block column j of width jb (reference structure:
+// library/src/lapack/roclapack_getrf.hpp:608-752, without rocBLAS):
+//   1. panel_kernel: one workgroup per matrix factors A[j:m, j:j+jb] with
+//      partial pivoting (workgroup arg-max per column), swaps the pivot rows
+//      across all n columns (roclapack_getf2.hpp:41-99 semantics), then solves
+//      U12 = L11^-1 A12 for the columns right of the panel.
+//   2. gemm_sub_kernel: A22 -= L21 * U12 on MFMA (v_mfma_f64_16x16x4f64 /
+//      v_mfma_f32_16x16x4f32), 64x64 tiles.
+// getrs (roclapack_getrs.hpp:117-150): laswp, then for each 64-row diagonal
+// block a substitution kernel followed by an MFMA GEMM update of the rest.
+#include "internal.hpp"
+
+#include <climits>
+
+namespace rsa
+{
+
+constexpr int PANEL_NB = 32; // panel width
+constexpr int PANEL_THREADS = 256;
+constexpr int TRSM_NB = 64; // getrs diagonal block
+constexpr int64_t MAX_GRID_YZ = 65535; // grid y/z limit: larger batches are launched in chunks
+
+// ------------------------------------------------------------------ info
+__global__ void reset_info_kernel(int* info, int bc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < bc)
+        info[i] = 0;
+}
+
+rocblas_status reset_info(hipStream_t stream, int* info, int bc)
+{
+    if(bc <= 0)
+        return rocblas_status_success;
+    hipLaunchKernelGGL(reset_info_kernel, dim3(ceil_div(bc, 256)), dim3(256), 0, stream, info, bc);
+    RSA_LAUNCH_CHECK();
+    return rocblas_status_success;
+}
+
+// ------------------------------------------------------------ panel + trsm
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T& key, int& idx)
+{
+#pragma unroll
+    for(int off = 1; off < 64; off <<= 1)
+    {
+        T ok = __shfl_xor(key, off, 64);
+        int oi = __shfl_xor(idx, off, 64);
+        if(ok > key || (ok == key && oi < idx))
+        {
+            key = ok;
+            idx = oi;
+        }
+    }
+}
+
+template <typename T, bool PIVOT>
+__global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batch<T> AA, int lda,
+                                                              int* __restrict__ ipivA,
+                                                              int64_t strideP,
+                                                              int* __restrict__ info, int j0,
+                                                              int jb)
+{
+    constexpr int NW = PANEL_THREADS / 64;
+    __shared__ T s_key[NW];
+    __shared__ int s_idx[NW];
+    __shared__ T s_pv;
+    __shared__ int s_p;
+    __shared__ T s_L[PANEL_NB * PANEL_NB];
+
+    const int64_t b = blockIdx.x;
+    T* A = AA[b];
+    const int tid = threadIdx.x;
+    const int wid = tid / 64;
+    const int lane = tid % 64;
+    auto at = [&](int i, int j) -> T& { return A[i + int64_t(j) * lda]; };
+
+    for(int k = 0; k < jb; ++k)
+    {
+        const int c = j0 + k;
+        if(PIVOT)
+        {
+            T key = T(-1);
+            int idx = INT_MAX;
+            for(int i = c + tid; i < m; i += PANEL_THREADS)
+            {
+                T kk = pivot_key(at(i, c), i == c);
+                if(kk > key)
+                {
+                    key = kk;
+                    idx = i;
+                }
+            }
+            wave_argmax(key, idx);
+            if(lane == 0)
+            {
+                s_key[wid] = key;
+                s_idx[wid] = idx;
+            }
+            __syncthreads();
+            if(tid == 0)
+            {
+                T bk = s_key[0];
+                int bi = s_idx[0];
+                for(int w = 1; w < NW; ++w)
+                    if(s_key[w] > bk || (s_key[w] == bk && s_idx[w] < bi))
+                    {
+                        bk = s_key[w];
+                        bi = s_idx[w];
+                    }
+                s_p = bi;
+                s_pv = at(bi, c);
+                ipivA[b * strideP + c] = bi + 1;
+                if(s_pv == T(0) && info[b] == 0)
+                    info[b] = c + 1;
+            }
+        }
+        else if(tid == 0)
+        {
+            s_p = c;
+            s_pv = at(c, c);
+            if(s_pv == T(0) && info[b] == 0)
+                info[b] = c + 1;
+        }
+        __syncthreads();
+        const int p = s_p;
+        const T pv = s_pv;
+        const bool nz = pv != T(0);
+        // interchange rows c and p across all n columns
+        if(nz && p != c)
+        {
+            for(int j = tid; j < n; j += PANEL_THREADS)
+            {
+                T t = at(c, j);
+                at(c, j) = at(p, j);
+                at(p, j) = t;
+            }
+        }
+        __syncthreads();
+        // scale the column and update the rest of the panel
+        const int jend = j0 + jb;
+        const T rcp = nz ? T(1) / pv : T(1);
+        for(int i = c + 1 + tid; i < m; i += PANEL_THREADS)
+        {
+            T l = at(i, c);
+            if(nz)
+            {
+                l *= rcp;
+                at(i, c) = l;
+            }
+            for(int j = c + 1; j < jend; ++j)
+                at(i, j) = fma_t(-l, at(c, j), at(i, j));
+        }
+        __syncthreads();
+    }
+
+    // U12 = L11^-1 A12 for the columns right of the panel
+    const int c0 = j0 + jb;
+    if(c0 < n)
+    {
+        for(int e = tid; e < jb * jb; e += PANEL_THREADS)
+        {
+            const int i = e % jb, k = e / jb;
+            s_L[i + k * PANEL_NB] = at(j0 + i, j0 + k);
+        }
+        __syncthreads();
+        for(int j = c0 + tid; j < n; j += PANEL_THREADS)
+        {
+            T x[PANEL_NB];
+#pragma unroll
+            for(int i = 0; i < PANEL_NB; ++i)
+                x[i] = i < jb ? at(j0 + i, j) : T(0);
+#pragma unroll
+            for(int k = 0; k < PANEL_NB; ++k)
+            {
+#pragma unroll
+                for(int i = k + 1; i < PANEL_NB; ++i)
+                    x[i] = fma_t(-s_L[i + k * PANEL_NB], x[k], x[i]);
+            }
+#pragma unroll
+            for(int i = 0; i < PANEL_NB; ++i)
+                if(i < jb)
+                    at(j0 + i, j) = x[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ GEMM
+// C[M x N] -= op(A)[M x K] * B[K x N], column-major, one 64x64 tile per
+// workgroup (4 waves, each 32x32 = 2x2 MFMA 16x16 blocks), K staged through
+// LDS in chunks of 16.  MFMA 16x16x4 operand layout (lane l): A[l%16][l/16],
+// B[l/16][l%16].  Result register r of lane l holds column l%16 and row
+// 4*(l/16)+r for the f32 instruction but row l/16+4*r for the f64 one
+// (measured with a one-instruction probe on gfx950); mfma16<T>::row gives it.
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct mfma16;
+template <>
+struct mfma16<double>
+{
+    typedef d4_t acc_t;
+    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r)
+    {
+        return lane / 16 + 4 * r;
+    }
+};
+template <>
+struct mfma16<float>
+{
+    typedef f4_t acc_t;
+    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r)
+    {
+        return 4 * (lane / 16) + r;
+    }
+};
+
+constexpr int GT = 64; // tile
+constexpr int GK = 16; // k chunk
+
+template <typename T, bool TRANSA>
+__global__ __launch_bounds__(256) void gemm_sub_kernel(int M, int N, int K, Batch<T> AA,
+                                                       int64_t offA, int lda, Batch<T> BB,
+                                                       int64_t offB, int ldb, Batch<T> CC,
+                                                       int64_t offC, int ldc,
+                                                       const int* __restrict__ skip, int64_t boff)
+{
+    using acc_t = typename mfma16<T>::acc_t;
+    __shared__ T sA[GK][GT]; // sA[k][i]
+    __shared__ T sB[GK][GT]; // sB[k][j]
+
+    const int64_t b = blockIdx.z + boff;
+    if(skip && skip[b] != 0)
+        return;
+    const T* A = AA[b] + offA;
+    const T* Bm = BB[b] + offB;
+    T* C = CC[b] + offC;
+    const int i0 = blockIdx.x * GT;
+    const int j0 = blockIdx.y * GT;
+    const int tid = threadIdx.x;
+    const int wid = tid / 64;
+    const int lane = tid % 64;
+    const int wi = (wid % 2) * 32;
+    const int wj = (wid / 2) * 32;
+
+    acc_t acc[2][2];
+#pragma unroll
+    for(int s = 0; s < 2; ++s)
+#pragma unroll
+        for(int t = 0; t < 2; ++t)
+            acc[s][t] = acc_t{0, 0, 0, 0};
+
+    for(int k0 = 0; k0 < K; k0 += GK)
+    {
+        // stage A: GT x GK, B: GK x GT (zero padded)
+#pragma unroll
+        for(int e = tid; e < GT * GK; e += 256)
+        {
+            int i, k;
+            T v;
+            if(!TRANSA)
+            {
+                i = e % GT;
+                k = e / GT;
+                v = (i0 + i < M && k0 + k < K) ? A[(i0 + i) + int64_t(k0 + k) * lda] : T(0);
+            }
+            else
+            {
+                k = e % GK;
+                i = e / GK;
+                v = (i0 + i < M && k0 + k < K) ? A[(k0 + k) + int64_t(i0 + i) * lda] : T(0);
+            }
+            sA[k][i] = v;
+            const int kb = e % GK, jb = e / GK;
+            sB[kb][jb] = (k0 + kb < K && j0 + jb < N) ? Bm[(k0 + kb) + int64_t(j0 + jb) * ldb]
+                                                      : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for(int kk = 0; kk < GK; kk += 4)
+        {
+            T a[2], bv[2];
+#pragma unroll
+            for(int s = 0; s < 2; ++s)
+                a[s] = sA[kk + lane / 16][wi + s * 16 + lane % 16];
+#pragma unroll
+            for(int t = 0; t < 2; ++t)
+                bv[t] = sB[kk + lane / 16][wj + t * 16 + lane % 16];
+#pragma unroll
+            for(int s = 0; s < 2; ++s)
+#pragma unroll
+                for(int t = 0; t < 2; ++t)
+                    acc[s][t] = mfma16<T>::run(a[s], bv[t], acc[s][t]);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for(int s = 0; s < 2; ++s)
+#pragma unroll
+        for(int t = 0; t < 2; ++t)
+        {
+            const int j = j0 + wj + t * 16 + lane % 16;
+#pragma unroll
+            for(int r = 0; r < 4; ++r)
+            {
+                const int i = i0 + wi + s * 16 + mfma16<T>::row(lane, r);
+                if(i < M && j < N)
+                {
+                    T& c = C[i + int64_t(j) * ldc];
+                    c = c - acc[s][t][r];
+                }
+            }
+        }
+}
+
+template <typename T>
+static rocblas_status gemm_sub(hipStream_t stream, bool transA, int M, int N, int K, Batch<T> A,
+                               int64_t offA, int lda, Batch<T> B, int64_t offB, int ldb,
+                               Batch<T> C, int64_t offC, int ldc, int bc, const int* skip)
+{
+    if(M <= 0 || N <= 0 || K <= 0 || bc <= 0)
+        return rocblas_status_success;
+    for(int64_t b0 = 0; b0 < bc; b0 += MAX_GRID_YZ)
+    {
+        const int nb = int((bc - b0) < MAX_GRID_YZ ? (bc - b0) : MAX_GRID_YZ);
+        dim3 grid(ceil_div(M, GT), ceil_div(N, GT), nb);
+        if(transA)
+            hipLaunchKernelGGL((gemm_sub_kernel<T, true>), grid, dim3(256), 0, stream, M, N, K, A,
+                               offA, lda, B, offB, ldb, C, offC, ldc, skip, b0);
+        else
+            hipLaunchKernelGGL((gemm_sub_kernel<T, false>), grid, dim3(256), 0, stream, M, N, K,
+                               A, offA, lda, B, offB, ldb, C, offC, ldc, skip, b0);
+        RSA_LAUNCH_CHECK();
+    }
+    return rocblas_status_success;
+}
+
+// ----------------------------------------------------------------- getrf
+template <typename T>
+rocblas_status getrf_blocked(hipStream_t stream, int m, int n, Batch<T> A, int lda, int* ipiv,
+                             int64_t strideP, int* info, int bc, bool pivot)
+{
+    rocblas_status st = reset_info(stream, info, bc);
+    if(st != rocblas_status_success)
+        return st;
+    const int kmax = m < n ? m : n;
+    for(int j = 0; j < kmax; j += PANEL_NB)
+    {
+        const int jb = (kmax - j) < PANEL_NB ? (kmax - j) : PANEL_NB;
+        if(pivot)
+            hipLaunchKernelGGL((panel_kernel<T, true>), dim3(bc), dim3(PANEL_THREADS), 0, stream,
+                               m, n, A, lda, ipiv, strideP, info, j, jb);
+        else
+            hipLaunchKernelGGL((panel_kernel<T, false>), dim3(bc), dim3(PANEL_THREADS), 0, stream,
+                               m, n, A, lda, ipiv, strideP, info, j, jb);
+        RSA_LAUNCH_CHECK();
+        const int r0 = j + jb;
+        if(r0 < m && r0 < n)
+        {
+            st = gemm_sub<T>(stream, false, m - r0, n - r0, jb, A, r0 + int64_t(j) * lda, lda, A,
+                             j + int64_t(r0) * lda, lda, A, r0 + int64_t(r0) * lda, lda, bc,
+                             nullptr);
+            if(st != rocblas_status_success)
+                return st;
+        }
+    }
+    return rocblas_status_success;
+}
+
+// ----------------------------------------------------------------- laswp
+// One thread per column; the interchanges of one column run in order
+// (rocauxiliary_laswp.hpp:41-62).
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_kernel(int ncols, Batch<T> AA, int lda, int k1,
+                                                    int k2, const int* __restrict__ ipivA,
+                                                    int64_t strideP, int incx,
+                                                    const int* __restrict__ skip, int64_t boff)
+{
+    const int64_t b = blockIdx.y + boff;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if(c >= ncols || (skip && skip[b] != 0))
+        return;
+    T* A = AA[b] + int64_t(c) * lda;
+    const int* ipiv = ipivA + b * strideP;
+    int start, end, inc;
+    if(incx < 0)
+    {
+        start = k2;
+        end = k1 - 1;
+        inc = -1;
+        incx = -incx;
+    }
+    else
+    {
+        start = k1;
+        end = k2 + 1;
+        inc = 1;
+    }
+    for(int i = start; i != end; i += inc)
+    {
+        const int ex = ipiv[k1 + (i - k1) * incx - 1];
+        // a pivot outside the column's storage is ignored instead of addressed
+        if(ex != i && ex >= 1 && ex <= lda)
+        {
+            T t = A[i - 1];
+            A[i - 1] = A[ex - 1];
+            A[ex - 1] = t;
+        }
+    }
+}
+
+template <typename T>
+rocblas_status laswp(hipStream_t stream, int ncols, Batch<T> A, int lda, int k1, int k2,
+                     const int* ipiv, int64_t strideP, int incx, int bc, const int* skip_info)
+{
+    if(ncols <= 0 || bc <= 0)
+        return rocblas_status_success;
+    for(int64_t b0 = 0; b0 < bc; b0 += MAX_GRID_YZ)
+    {
+        const int nb = int((bc - b0) < MAX_GRID_YZ ? (bc - b0) : MAX_GRID_YZ);
+        dim3 grid(ceil_div(ncols, 256), nb);
+        hipLaunchKernelGGL((laswp_kernel<T>), grid, dim3(256), 0, stream, ncols, A, lda, k1, k2,
+                           ipiv, strideP, incx, skip_info, b0);
+        RSA_LAUNCH_CHECK();
+    }
+    return rocblas_status_success;
+}
+
+// ------------------------------------------------------------ substitution
+// Solves one diagonal block (rows r0..r0+rb) of the LU against all right-hand
+// sides, one thread per column of B, block in LDS.  kind: 0 = L unit lower
+// (forward), 1 = U non-unit upper (backward), 2 = U^T (forward, non-unit),
+// 3 = L^T (backward, unit).  Column-oriented order with true division, as
+// the reference substitution kernels.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void trsm_block_kernel(int r0, int rb, int nrhs, Batch<T> AA,
+                                                         int lda, Batch<T> BB, int ldb,
+                                                         const int* __restrict__ skip,
+                                                         int64_t boff)
+{
+    __shared__ T sD[TRSM_NB * TRSM_NB];
+    const int64_t b = blockIdx.y + boff;
+    if(skip && skip[b] != 0)
+        return;
+    const T* A = AA[b];
+    T* Bm = BB[b];
+    const int tid = threadIdx.x;
+    for(int e = tid; e < rb * rb; e += 256)
+    {
+        const int i = e % rb, k = e / rb;
+        sD[i + k * TRSM_NB] = A[(r0 + i) + int64_t(r0 + k) * lda];
+    }
+    __syncthreads();
+    const int c = blockIdx.x * 256 + tid;
+    if(c >= nrhs)
+        return;
+    T x[TRSM_NB];
+#pragma unroll
+    for(int i = 0; i < TRSM_NB; ++i)
+        x[i] = i < rb ? Bm[(r0 + i) + int64_t(c) * ldb] : T(0);
+    auto D = [&](int i, int k) { return sD[i + k * TRSM_NB]; };
+    if(KIND == 0)
+    {
+#pragma unroll
+        for(int k = 0; k < TRSM_NB; ++k)
+#pragma unroll
+            for(int i = k + 1; i < TRSM_NB; ++i)
+                if(i < rb)
+                    x[i] = fma_t(-D(i, k), x[k], x[i]);
+    }
+    else if(KIND == 1)
+    {
+#pragma unroll
+        for(int k = TRSM_NB - 1; k >= 0; --k)
+        {
+            if(k < rb)
+            {
+                x[k] = x[k] / D(k, k);
+#pragma unroll
+                for(int i = 0; i < k; ++i)
+                    x[i] = fma_t(-D(i, k), x[k], x[i]);
+            }
+        }
+    }
+    else if(KIND == 2)
+    {
+#pragma unroll
+        for(int k = 0; k < TRSM_NB; ++k)
+        {
+            if(k < rb)
+            {
+                x[k] = x[k] / D(k, k);
+#pragma unroll
+                for(int i = k + 1; i < TRSM_NB; ++i)
+                    if(i < rb)
+                        x[i] = fma_t(-D(k, i), x[k], x[i]);
+            }
+        }
+    }
+    else
+    {
+#pragma unroll
+        for(int k = TRSM_NB - 1; k > 0; --k)
+        {
+            if(k < rb)
+            {
+#pragma unroll
+                for(int i = 0; i < k; ++i)
+                    x[i] = fma_t(-D(k, i), x[k], x[i]);
+            }
+        }
+    }
+#pragma unroll
+    for(int i = 0; i < TRSM_NB; ++i)
+        if(i < rb)
+            Bm[(r0 + i) + int64_t(c) * ldb] = x[i];
+}
+
+template <typename T, int KIND>
+static rocblas_status trsm_block(hipStream_t stream, int r0, int rb, int nrhs, Batch<T> A, int lda,
+                                 Batch<T> B, int ldb, int bc, const int* skip)
+{
+    for(int64_t b0 = 0; b0 < bc; b0 += MAX_GRID_YZ)
+    {
+        const int nb = int((bc - b0) < MAX_GRID_YZ ? (bc - b0) : MAX_GRID_YZ);
+        dim3 grid(ceil_div(nrhs, 256), nb);
+        hipLaunchKernelGGL((trsm_block_kernel<T, KIND>), grid, dim3(256), 0, stream, r0, rb, nrhs,
+                           A, lda, B, ldb, skip, b0);
+        RSA_LAUNCH_CHECK();
+    }
+    return rocblas_status_success;
+}
+
+template <typename T>
+rocblas_status getrs_blocked(hipStream_t stream, bool trans, int n, int nrhs, Batch<T> A, int lda,
+                             const int* ipiv, int64_t strideP, Batch<T> B, int ldb, int bc,
+                             bool pivot, const int* skip)
+{
+    rocblas_status st = rocblas_status_success;
+#define RSA_TRY(x)                            \
+    do                                        \
+    {                                         \
+        st = (x);                             \
+        if(st != rocblas_status_success)      \
+            return st;                        \
+    } while(0)
+    const int nb = TRSM_NB;
+    if(!trans)
+    {
+        if(pivot)
+            RSA_TRY(laswp<T>(stream, nrhs, B, ldb, 1, n, ipiv, strideP, 1, bc, skip));
+        for(int r0 = 0; r0 < n; r0 += nb)
+        {
+            const int rb = (n - r0) < nb ? (n - r0) : nb;
+            RSA_TRY((trsm_block<T, 0>(stream, r0, rb, nrhs, A, lda, B, ldb, bc, skip)));
+            const int r1 = r0 + rb;
+            RSA_TRY(gemm_sub<T>(stream, false, n - r1, nrhs, rb, A, r1 + int64_t(r0) * lda, lda, B,
+                                r0, ldb, B, r1, ldb, bc, skip));
+        }
+        const int last = ((n - 1) / nb) * nb;
+        for(int r0 = last; r0 >= 0; r0 -= nb)
+        {
+            const int rb = (n - r0) < nb ? (n - r0) : nb;
+            RSA_TRY((trsm_block<T, 1>(stream, r0, rb, nrhs, A, lda, B, ldb, bc, skip)));
+            RSA_TRY(gemm_sub<T>(stream, false, r0, nrhs, rb, A, int64_t(r0) * lda, lda, B, r0,
+                                ldb, B, 0, ldb, bc, skip));
+        }
+    }
+    else
+    {
+        for(int r0 = 0; r0 < n; r0 += nb)
+        {
+            const int rb = (n - r0) < nb ? (n - r0) : nb;
+            RSA_TRY((trsm_block<T, 2>(stream, r0, rb, nrhs, A, lda, B, ldb, bc, skip)));
+            const int r1 = r0 + rb;
+            // B[r1:n] -= (U[r0:r1, r1:n])^T B[r0:r1]
+            RSA_TRY(gemm_sub<T>(stream, true, n - r1, nrhs, rb, A, r0 + int64_t(r1) * lda, lda, B,
+                                r0, ldb, B, r1, ldb, bc, skip));
+        }
+        const int last = ((n - 1) / nb) * nb;
+        for(int r0 = last; r0 >= 0; r0 -= nb)
+        {
+            const int rb = (n - r0) < nb ? (n - r0) : nb;
+            RSA_TRY((trsm_block<T, 3>(stream, r0, rb, nrhs, A, lda, B, ldb, bc, skip)));
+            // B[0:r0] -= (L[r0:r1, 0:r0])^T B[r0:r1]
+            RSA_TRY(gemm_sub<T>(stream, true, r0, nrhs, rb, A, r0, lda, B, r0, ldb, B, 0, ldb, bc,
+                                skip));
+        }
+        if(pivot)
+            RSA_TRY(laswp<T>(stream, nrhs, B, ldb, 1, n, ipiv, strideP, -1, bc, skip));
+    }
+#undef RSA_TRY
+    return rocblas_status_success;
+}
+
+#define RSA_INST(T)                                                                              \
+    template rocblas_status getrf_blocked<T>(hipStream_t, int, int, Batch<T>, int, int*, int64_t, \
+                                             int*, int, bool);                                   \
+    template rocblas_status getrs_blocked<T>(hipStream_t, bool, int, int, Batch<T>, int,          \
+                                             const int*, int64_t, Batch<T>, int, int, bool,       \
+                                             const int*);                                        \
+    template rocblas_status laswp<T>(hipStream_t, int, Batch<T>, int, int, int, const int*,       \
+                                     int64_t, int, int, const int*);
+RSA_INST(float)
+RSA_INST(double)
+#undef RSA_INST
+
+} // namespace rsa

@@ -1,0 +1,94 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares (dlopen RTLD_NOW must resolve everything, as
+clients/extras/test_dlopen_main.cpp:11-14 checks for the reference), and the
+argument checks that return before the handle is first used give the
+reference's status codes in the reference's order."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import rocsolver_amd as ra
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "rocsolver_amd.h")
+DUMMY = 0x1000  # non-null handle value; these checks return before it is dereferenced
+
+
+def header_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\brocblas_status\s+(rocsolver_\w+)\s*\(", text)))
+
+
+def test_library_loads_with_rtld_now():
+    ctypes.CDLL(ra.LIB_PATH, mode=os.RTLD_NOW)
+
+
+def test_every_declared_symbol_is_exported():
+    L = ctypes.CDLL(ra.LIB_PATH)
+    syms = header_symbols()
+    assert len(syms) >= 34
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(ra.exported_symbols()) <= set(syms)
+
+
+def test_version_string():
+    v = ra.version()
+    assert v.startswith("3.23.0")
+    n = ctypes.c_size_t()
+    assert ra.lib().rocsolver_get_version_string_size(ctypes.byref(n)) == 0
+    assert n.value == len(v) + 1
+    assert ra.lib().rocsolver_get_version_string(None, 10) == 3
+    buf = ctypes.create_string_buffer(2)
+    assert ra.lib().rocsolver_get_version_string(buf, 2) == 4
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_null_handle(p):
+    assert ra.call(f"rocsolver_{p}getrf_strided_batched", None, 4, 4, DUMMY, 4, 16, DUMMY, 4,
+                   DUMMY, 1) == 1
+    assert ra.call(f"rocsolver_{p}getrs_strided_batched", None, ra.OP_NONE, 4, 1, DUMMY, 4, 16,
+                   DUMMY, 4, DUMMY, 4, 4, 1) == 1
+    assert ra.call(f"rocsolver_{p}gesv_strided_batched", None, 4, 1, DUMMY, 4, 16, DUMMY, 4,
+                   DUMMY, 4, 4, DUMMY, 1) == 1
+    assert ra.call(f"rocsolver_{p}getrf_batched", None, 4, 4, DUMMY, 4, DUMMY, 4, DUMMY, 1) == 1
+    assert ra.call(f"rocsolver_{p}gesv_batched", None, 4, 1, DUMMY, 4, DUMMY, 4, DUMMY, 4,
+                   DUMMY, 1) == 1
+    assert ra.call(f"rocsolver_{p}laswp", None, 4, DUMMY, 4, 1, 4, DUMMY, 1) == 1
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+@pytest.mark.parametrize("m,n,lda,bc", [(-1, 4, 4, 1), (4, -1, 4, 1), (20, 4, 5, 1),
+                                        (4, 4, 4, -1)])
+def test_getrf_invalid_size(p, m, n, lda, bc):
+    # roclapack_getf2.hpp:502-503: m<0 || n<0 || lda<m || bc<0
+    for name in ("getrf_strided_batched", "getf2_strided_batched"):
+        assert ra.call(f"rocsolver_{p}{name}", DUMMY, m, n, DUMMY, lda, 16, DUMMY, 4, DUMMY,
+                       bc) == 4
+    assert ra.call(f"rocsolver_{p}getrf_npvt_strided_batched", DUMMY, m, n, DUMMY, lda, 16,
+                   DUMMY, bc) == 4
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_getrs_value_before_size(p):
+    # roclapack_getrs.hpp:32-38: invalid trans is reported before invalid sizes
+    assert ra.call(f"rocsolver_{p}getrs_strided_batched", DUMMY, 999, -1, 1, DUMMY, 4, 16, DUMMY,
+                   4, DUMMY, 4, 4, 1) == 11
+    for n, nrhs, lda, ldb, bc in [(-1, 1, 4, 4, 1), (4, -1, 4, 4, 1), (10, 1, 2, 10, 1),
+                                  (10, 1, 10, 2, 1), (4, 1, 4, 4, -1)]:
+        assert ra.call(f"rocsolver_{p}getrs_strided_batched", DUMMY, ra.OP_NONE, n, nrhs, DUMMY,
+                       lda, 16, DUMMY, 4, DUMMY, ldb, 4, bc) == 4
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_gesv_and_laswp_invalid_size(p):
+    for n, nrhs, lda, ldb, bc in [(-1, 1, 4, 4, 1), (4, -1, 4, 4, 1), (10, 1, 2, 10, 1),
+                                  (10, 1, 10, 2, 1), (4, 1, 4, 4, -1)]:
+        assert ra.call(f"rocsolver_{p}gesv_strided_batched", DUMMY, n, nrhs, DUMMY, lda, 16,
+                       DUMMY, 4, DUMMY, ldb, 4, DUMMY, bc) == 4
+    # rocauxiliary_laswp.hpp:82-83: n<0 || lda<1 || !incx || k1<1 || k2<1 || k2<k1
+    for n, lda, k1, k2, incx in [(-1, 4, 1, 2, 1), (4, 0, 1, 2, 1), (4, 4, 1, 2, 0),
+                                 (4, 4, 0, 2, 1), (4, 4, 1, 0, 1), (4, 4, 3, 2, 1)]:
+        assert ra.call(f"rocsolver_{p}laswp", DUMMY, n, DUMMY, lda, k1, k2, DUMMY, incx) == 4

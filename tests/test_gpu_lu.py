@@ -1,0 +1,272 @@
+"""GPU parity of getrf / getrs / gesv through the C ABI against the oracle.
+
+Contract (reference tests clients/include/testing_getf2_getrf.hpp:237-255,
+testing_getrs.hpp:216, testing_gesv.hpp:237-245): ipiv and info exactly equal
+to the host result, LU within min(m,n)*eps (relative Frobenius), X within
+n*eps (relative inf-norm).  The register-resident kernels (m, n <= 64) run the
+oracle's exact operation sequence, so there the LU and X are bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rocsolver_amd as ra
+from rocsolver_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps}
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def rel_fro(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+GETRF_SIZES = [(m, lda, s, n) for (m, lda, s) in [(32, 32, 0), (50, 50, 1), (70, 100, 0)]
+               for n in (16, 20, 40, 100)]
+
+
+@pytest.mark.parametrize("m,lda,singular,n", GETRF_SIZES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("batched", [False, True])
+def test_getrf_reference_matrices(torch_gpu, handle, m, lda, singular, n, dtype, batched):
+    print(f"getrf m={m} n={n} lda={lda} {dtype.__name__} batched={batched}", flush=True)
+    A0 = wl.reference_getrf(m, n, lda, 3, bool(singular), dtype)
+    Ao = A0.copy()
+    ip_o, in_o = oracle.getf2(Ao, m)
+    Ad = dev(torch_gpu, A0)
+    ip, info = ra.getrf(handle, Ad, m=m, batched=batched)
+    torch_gpu.cuda.synchronize()
+    np.testing.assert_array_equal(host(ip), ip_o)
+    np.testing.assert_array_equal(host(info), in_o)
+    Ag = host(Ad)
+    if m <= 64 and n <= 64:
+        np.testing.assert_array_equal(Ag, Ao)  # same operation sequence -> bit-identical
+    for b in range(3):
+        assert rel_fro(Ag[b, :n, :m], Ao[b, :n, :m]) <= min(m, n) * EPS[dtype]
+    # rows beyond m (lda padding) are untouched
+    np.testing.assert_array_equal(Ag[:, :, m:], A0[:, :, m:])
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 7), (7, 3), (16, 16), (33, 33), (64, 64), (64, 8),
+                                 (8, 64), (65, 65), (96, 96), (128, 128), (130, 70), (70, 130),
+                                 (256, 256), (300, 300)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_getrf_random(torch_gpu, handle, m, n, dtype):
+    print(f"getrf random m={m} n={n} {dtype.__name__}", flush=True)
+    g = np.random.default_rng(m * 7 + n)
+    bc = 5
+    A0 = g.uniform(-1, 1, size=(bc, n, m)).astype(dtype)
+    Ao = A0.copy()
+    ip_o, in_o, gap = oracle.getf2(Ao, m, with_gap=True)
+    Ad = dev(torch_gpu, A0)
+    ip, info = ra.getrf(handle, Ad, m=m)
+    torch_gpu.cuda.synchronize()
+    Ag = host(Ad)
+    ipg = host(ip)
+    np.testing.assert_array_equal(host(info), in_o)
+    unamb = (gap > 1e3 * max(m, n) * EPS[dtype]).all(axis=1)
+    np.testing.assert_array_equal(ipg[unamb], ip_o[unamb])
+    if m <= 64 and n <= 64:
+        np.testing.assert_array_equal(Ag, Ao)
+        np.testing.assert_array_equal(ipg, ip_o)
+    be = oracle.lu_backward_error(A0.astype(np.float64), Ag.astype(np.float64), ipg, m, n,
+                                  EPS[dtype])
+    assert be.max() < 10.0, be
+    for b in np.nonzero(unamb)[0]:
+        assert rel_fro(Ag[b], Ao[b]) <= 10 * min(m, n) * EPS[dtype]
+
+
+@pytest.mark.parametrize("n", [4, 32, 100])
+def test_getrf_npvt(torch_gpu, handle, n):
+    g = np.random.default_rng(n)
+    A0 = g.uniform(-1, 1, size=(3, n, n)) + n * np.eye(n)[None]
+    Ad = dev(torch_gpu, A0)
+    _, info = ra.getrf(handle, Ad, pivot=False)
+    torch_gpu.cuda.synchronize()
+    Ag = host(Ad)
+    assert (host(info) == 0).all()
+    ident = np.tile(np.arange(1, n + 1, dtype=np.int32), (3, 1))
+    be = oracle.lu_backward_error(A0, Ag, ident, n, n, EPS[np.float64])
+    assert be.max() < 10.0
+
+
+@pytest.mark.parametrize("n,lda,ldb,nrhs", [(20, 20, 20, 10), (30, 50, 30, 20), (30, 30, 50, 30),
+                                            (50, 60, 60, 1), (64, 64, 64, 3), (70, 70, 100, 2),
+                                            (130, 130, 130, 9), (192, 192, 200, 1)])
+@pytest.mark.parametrize("trans", ["N", "T"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_getrs(torch_gpu, handle, n, lda, ldb, nrhs, trans, dtype):
+    print(f"getrs n={n} nrhs={nrhs} {trans} {dtype.__name__}", flush=True)
+    A, B = wl.reference_getrs(n, nrhs, lda, ldb, 3, dtype)
+    ipiv, _ = oracle.getf2(A, n)  # host LU first, as the reference test does
+    Xo = B.copy()
+    oracle.getrs(trans, A, n, ipiv, Xo)
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    ipd = dev(torch_gpu, ipiv)
+    ra.getrs(handle, Ad, ipd, Bd, trans=trans)
+    torch_gpu.cuda.synchronize()
+    Xg = host(Bd)
+    if n <= 64:
+        np.testing.assert_array_equal(Xg, Xo)
+    for b in range(3):
+        d = np.abs(Xg[b, :, :n] - Xo[b, :, :n]).max() / np.abs(Xo[b, :, :n]).max()
+        assert d <= n * EPS[dtype]
+    np.testing.assert_array_equal(Xg[:, :, n:], B[:, :, n:])  # ldb padding untouched
+
+
+@pytest.mark.parametrize("n,lda,ldb,singular,nrhs", [(20, 20, 20, 0, 10), (30, 50, 30, 1, 20),
+                                                     (30, 30, 50, 0, 30), (50, 60, 60, 1, 1),
+                                                     (70, 70, 100, 0, 100), (100, 100, 100, 1, 8),
+                                                     (192, 192, 192, 1, 4)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("batched", [False, True])
+def test_gesv(torch_gpu, handle, n, lda, ldb, singular, nrhs, dtype, batched):
+    print(f"gesv n={n} nrhs={nrhs} {dtype.__name__} batched={batched}", flush=True)
+    A, B = wl.reference_gesv(n, nrhs, lda, ldb, 4, bool(singular), dtype)
+    Ao, Xo = A.copy(), B.copy()
+    ip_o, in_o = oracle.gesv(Ao, n, Xo)
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    ip, info = ra.gesv(handle, Ad, Bd, batched=batched)
+    torch_gpu.cuda.synchronize()
+    np.testing.assert_array_equal(host(info), in_o)
+    Xg = host(Bd)
+    for b in range(4):
+        if in_o[b] != 0:
+            np.testing.assert_array_equal(Xg[b], B[b])  # roclapack_gesv.hpp:162-175
+        else:
+            d = np.abs(Xg[b, :, :n] - Xo[b, :, :n]).max() / np.abs(Xo[b, :, :n]).max()
+            assert d <= n * EPS[dtype]
+    if n <= 64:
+        np.testing.assert_array_equal(Xg, Xo)
+        np.testing.assert_array_equal(host(Ad), Ao)
+        np.testing.assert_array_equal(host(ip), ip_o)
+
+
+def test_invalid_pointers_and_quick_returns(torch_gpu, handle):
+    import ctypes
+    torch = torch_gpu
+    A = torch.zeros((2, 4, 4), dtype=torch.float64, device="cuda")
+    ipiv = torch.zeros((2, 4), dtype=torch.int32, device="cuda")
+    info = torch.full((2,), 7, dtype=torch.int32, device="cuda")
+    h = handle.value
+    f = "rocsolver_dgetrf_strided_batched"
+    assert ra.call(f, h, 4, 4, None, 4, 16, ipiv.data_ptr(), 4, info.data_ptr(), 2) == 3
+    assert ra.call(f, h, 4, 4, A.data_ptr(), 4, 16, None, 4, info.data_ptr(), 2) == 3
+    assert ra.call(f, h, 4, 4, A.data_ptr(), 4, 16, ipiv.data_ptr(), 4, None, 2) == 3
+    # m*n == 0 allows null A and ipiv; info reset to 0
+    assert ra.call(f, h, 0, 4, None, 1, 16, None, 4, info.data_ptr(), 2) == 0
+    torch.cuda.synchronize()
+    assert (info == 0).all()
+    # bc == 0 allows null info
+    assert ra.call(f, h, 4, 4, A.data_ptr(), 4, 16, ipiv.data_ptr(), 4, None, 0) == 0
+    # npvt needs no ipiv
+    assert ra.call("rocsolver_dgetrf_npvt_strided_batched", h, 4, 4, A.data_ptr(), 4, 16,
+                   info.data_ptr(), 2) == 0
+    # getrs pointers (roclapack_getrs.hpp:44-46)
+    B = torch.zeros((2, 1, 4), dtype=torch.float64, device="cuda")
+    g = "rocsolver_dgetrs_strided_batched"
+    assert ra.call(g, h, ra.OP_NONE, 4, 1, None, 4, 16, ipiv.data_ptr(), 4, B.data_ptr(), 4, 4,
+                   2) == 3
+    assert ra.call(g, h, ra.OP_NONE, 4, 1, A.data_ptr(), 4, 16, None, 4, B.data_ptr(), 4, 4,
+                   2) == 3
+    assert ra.call(g, h, ra.OP_NONE, 4, 1, A.data_ptr(), 4, 16, ipiv.data_ptr(), 4, None, 4, 4,
+                   2) == 3
+    assert ra.call(g, h, ra.OP_NONE, 4, 0, A.data_ptr(), 4, 16, ipiv.data_ptr(), 4, None, 4, 4,
+                   2) == 0
+    # gesv: info reset even when nrhs == 0 (roclapack_gesv.hpp:145-152)
+    info.fill_(5)
+    assert ra.call("rocsolver_dgesv_strided_batched", h, 4, 0, A.data_ptr(), 4, 16,
+                   ipiv.data_ptr(), 4, None, 4, 4, info.data_ptr(), 2) == 0
+    torch.cuda.synchronize()
+    assert (info == 0).all()
+
+
+def test_size_query_mode(torch_gpu, handle):
+    import ctypes
+    rb = ra.rocblas()
+    h = handle.value
+    assert rb.rocblas_start_device_memory_size_query(h) == 0
+    # pointers are not checked in query mode (roclapack_getf2.hpp:506-508)
+    st = ra.call("rocsolver_dgetrf_strided_batched", h, 64, 64, None, 64, 4096, None, 64, None,
+                 10)
+    assert st in (9, 10)  # size_increased / size_unchanged
+    st = ra.call("rocsolver_dgesv_strided_batched", h, 64, 2, None, 64, 4096, None, 64, None, 64,
+                 128, None, 10)
+    assert st in (9, 10)
+    assert ra.call("rocsolver_dlaswp", h, 4, None, 4, 1, 4, None, 1) == 10
+    size = ctypes.c_size_t()
+    assert rb.rocblas_stop_device_memory_size_query(h, ctypes.byref(size)) == 0
+
+
+def test_laswp(torch_gpu, handle):
+    n, ncols = 40, 5
+    g = np.random.default_rng(2)
+    ipiv = np.array([g.integers(k + 1, n + 1) for k in range(n)], dtype=np.int32)
+    B = g.standard_normal((1, ncols, n))
+    for k1, k2, incx in [(1, n, 1), (3, 30, 1), (1, n, -1), (5, 9, -1)]:
+        Bo = B.copy()
+        oracle.laswp(Bo, k1, k2, ipiv[None], incx)
+        Bd = dev(torch_gpu, B)
+        ipd = dev(torch_gpu, ipiv)
+        ra.check("rocsolver_dlaswp", handle.value, ncols, Bd.data_ptr(), n, k1, k2,
+                 ipd.data_ptr(), incx)
+        torch_gpu.cuda.synchronize()
+        np.testing.assert_array_equal(host(Bd), Bo)
+
+
+def test_config1_full_batch_bit_exact(torch_gpu, handle):
+    """Workload of the n=32 benchmark (bc=1024, U(-1,1), 1 RHS): getrf + getrs
+    bit-identical to the oracle over the whole batch."""
+    A, B = wl.config1()
+    Ao, Xo = A.copy(), B.copy()
+    ip_o, in_o = oracle.getf2(Ao, 32)
+    oracle.getrs("N", Ao, 32, ip_o, Xo)
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    ip, info = ra.getrf(handle, Ad)
+    ra.getrs(handle, Ad, ip, Bd)
+    torch_gpu.cuda.synchronize()
+    np.testing.assert_array_equal(host(ip), ip_o)
+    np.testing.assert_array_equal(host(info), in_o)
+    np.testing.assert_array_equal(host(Ad), Ao)
+    np.testing.assert_array_equal(host(Bd), Xo)
+
+
+def test_config2_full_batch_properties(torch_gpu, handle):
+    """n=128 benchmark workload (bc=16384): backward error of every LU on the
+    GPU in FP64, exact ipiv/info against the oracle on a sample."""
+    torch = torch_gpu
+    A, B = wl.config2()
+    Ad = dev(torch, A)
+    ip, info = ra.getrf(handle, Ad)
+    torch.cuda.synchronize()
+    assert (host(info) == 0).all()
+    n = 128
+    # || P A - L U ||_F / (n ||A||_F eps) for every matrix, computed with torch
+    LU = Ad.transpose(1, 2)  # row-major view LU[b, i, j]
+    L = torch.tril(LU, -1) + torch.eye(n, dtype=LU.dtype, device="cuda")
+    U = torch.triu(LU)
+    A_rm = dev(torch, A).transpose(1, 2)
+    perm = torch.arange(n, device="cuda").repeat(A.shape[0], 1)
+    ipc = ip.long() - 1
+    for k in range(n):
+        pk = ipc[:, k]
+        a = perm[:, k].clone()
+        bb = perm.gather(1, pk[:, None])[:, 0]
+        perm[:, k] = bb
+        perm.scatter_(1, pk[:, None], a[:, None])
+    PA = A_rm.gather(1, perm[:, :, None].expand(-1, -1, n))
+    R = PA - torch.matmul(L, U)
+    be = torch.linalg.norm(R, dim=(1, 2)) / (n * torch.linalg.norm(A_rm, dim=(1, 2)) * EPS[np.float64])
+    assert float(be.max()) < 10.0
+    idx = np.arange(0, A.shape[0], 257)
+    As = A[idx].copy()
+    ip_o, in_o = oracle.getf2(As, n)
+    np.testing.assert_array_equal(host(ip)[idx], ip_o)
