@@ -51,6 +51,8 @@ rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool 
         return reset_info(stream, info, bc);
     if(m <= SMALL_MAX && n <= SMALL_MAX)
         return getrf_small<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
+    if(pivot && m == n && n <= FUSED_MAX)
+        return lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
     return getrf_blocked<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
 }
 
@@ -116,7 +118,9 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
         return reset_info(stream, info, bc);
     if(n <= SMALL_MAX)
         return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
-    rocblas_status st = getrf_blocked<T>(stream, n, n, A, lda, ipiv, strideP, info, bc, true);
+    rocblas_status st = n <= FUSED_MAX
+                            ? lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc)
+                            : getrf_blocked<T>(stream, n, n, A, lda, ipiv, strideP, info, bc, true);
     if(st != rocblas_status_success)
         return st;
     return getrs_blocked<T>(stream, false, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,

@@ -74,6 +74,24 @@ __device__ __forceinline__ T pivot_key(T a, bool is_first_row)
     return abs_t(a);
 }
 
+// Arg-max of (key, index) over the whole wavefront, ties to the smaller
+// index; every lane ends with the winner.
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T& key, int& idx)
+{
+#pragma unroll
+    for(int off = 1; off < 64; off <<= 1)
+    {
+        T ok = __shfl_xor(key, off, 64);
+        int oi = __shfl_xor(idx, off, 64);
+        if(ok > key || (ok == key && oi < idx))
+        {
+            key = ok;
+            idx = oi;
+        }
+    }
+}
+
 inline int ceil_div(int64_t a, int64_t b)
 {
     return int((a + b - 1) / b);

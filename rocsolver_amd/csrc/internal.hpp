@@ -25,6 +25,16 @@ rocblas_status getrs_small(hipStream_t stream, bool trans, int n, int nrhs, Batc
                            const int* ipiv, int64_t strideP, Batch<T> B, int ldb, int bc,
                            bool pivot);
 
+// ---------------------------------------------------------- fused LU path
+// Square LU for SMALL_MAX < n <= FUSED_MAX, one workgroup per matrix.  AI is
+// read (type TI), the factors are written to AO (type TC; AO may alias AI
+// when TI == TC).
+constexpr int FUSED_MAX = 256;
+
+template <typename TI, typename TC>
+rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO, int ldo,
+                        int* ipiv, int64_t strideP, int* info, int bc);
+
 // -------------------------------------------------------------- blocked path
 template <typename T>
 rocblas_status getrf_blocked(hipStream_t stream, int m, int n, Batch<T> A, int lda, int* ipiv,

@@ -12,6 +12,7 @@
 // getrs (roclapack_getrs.hpp:117-150): laswp, then for each 64-row diagonal
 // block a substitution kernel followed by an MFMA GEMM update of the rest.
 #include "internal.hpp"
+#include "mfma.hpp"
 
 #include <climits>
 
@@ -41,22 +42,6 @@ rocblas_status reset_info(hipStream_t stream, int* info, int bc)
 }
 
 // ------------------------------------------------------------ panel + trsm
-template <typename T>
-__device__ __forceinline__ void wave_argmax(T& key, int& idx)
-{
-#pragma unroll
-    for(int off = 1; off < 64; off <<= 1)
-    {
-        T ok = __shfl_xor(key, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        if(ok > key || (ok == key && oi < idx))
-        {
-            key = ok;
-            idx = oi;
-        }
-    }
-}
-
 template <typename T, bool PIVOT>
 __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batch<T> AA, int lda,
                                                               int* __restrict__ ipivA,
@@ -191,42 +176,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
 // ------------------------------------------------------------------ GEMM
 // C[M x N] -= op(A)[M x K] * B[K x N], column-major, one 64x64 tile per
 // workgroup (4 waves, each 32x32 = 2x2 MFMA 16x16 blocks), K staged through
-// LDS in chunks of 16.  MFMA 16x16x4 operand layout (lane l): A[l%16][l/16],
-// B[l/16][l%16].  Result register r of lane l holds column l%16 and row
-// 4*(l/16)+r for the f32 instruction but row l/16+4*r for the f64 one
-// (measured with a one-instruction probe on gfx950); mfma16<T>::row gives it.
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
-template <typename T>
-struct mfma16;
-template <>
-struct mfma16<double>
-{
-    typedef d4_t acc_t;
-    static __device__ __forceinline__ acc_t run(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r)
-    {
-        return lane / 16 + 4 * r;
-    }
-};
-template <>
-struct mfma16<float>
-{
-    typedef f4_t acc_t;
-    static __device__ __forceinline__ acc_t run(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int r)
-    {
-        return 4 * (lane / 16) + r;
-    }
-};
-
+// LDS in chunks of 16 (MFMA operand/result layouts: mfma.hpp).
 constexpr int GT = 64; // tile
 constexpr int GK = 16; // k chunk
 

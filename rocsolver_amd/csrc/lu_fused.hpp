@@ -1,0 +1,355 @@
+// lu_fused.hpp -- one-workgroup-per-matrix left-looking LU for 64 < n <= 256.
+//
+// The whole factorisation of a matrix runs in one launch and one workgroup
+// of NMAX threads (NMAX = 128 or 256), one matrix per workgroup, several
+// workgroups per CU.  Per block column s (width 32):
+//   1. gather: the block column is read from the input in its original row
+//      order (coalesced, staged through LDS) and placed in MFMA accumulators
+//      in the current logical row order;
+//   2. left-looking update, for every factored block column t < s:
+//        U_ts = L_tt^-1 C[t-rows]                 (substitution, 32 threads)
+//        C[rows below t] -= L21_t * U_ts          (MFMA 16x16x4, K staged in LDS)
+//   3. panel: the accumulators are transposed through LDS to one logical row
+//      per thread and factored with partial pivoting, one barrier per column:
+//      every wave publishes its best candidate row (arg-max, ties to the
+//      lower row -- the reference rule, see pivot_key) and all threads pick
+//      the winner;
+//   4. the finished block column is written to the output in the logical
+//      order of that moment.
+// Row interchanges never touch memory.  Rows of L21_t are left where step t
+// wrote them and later interchanges are recorded in a per-block map
+// (ssig[t][logical row] = storage row), used as a gather from LDS in step 2;
+// a last pass moves every L21_t into the final order, so the output is the
+// LAPACK layout (A = P L U, 1-based ipiv, info = first zero pivot).
+// Output may alias the input (in-place getrf) or be a separate buffer of
+// another precision (the FP32 factorisation of an FP64 matrix).
+#pragma once
+
+#include "internal.hpp"
+#include "mfma.hpp"
+
+namespace rsa
+{
+
+constexpr int FNB = 32; // block-column width
+constexpr int FHC = 16; // columns per LDS round of the staging/transposition buffer
+
+// At least 3 waves per SIMD: without the bound the compiler spreads into
+// ~300 registers (1 wave per SIMD); with it the kernel fits ~150 VGPRs and
+// no scratch, so 3 workgroups of 256 threads share a CU.
+template <typename TI, typename TC, int NMAX>
+__global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
+                                                        int ldo, int* __restrict__ ipivA,
+                                                        int64_t strideP, int* __restrict__ infoA,
+                                                        int bc)
+{
+    using M = mfma16<TC>;
+    using acc_t = typename M::acc_t;
+    constexpr int NW = NMAX / 64;
+    constexpr int SMAX = NMAX / FNB;
+    constexpr int SB = FHC + 1; // padded row stride of the row-major view
+    constexpr int UB = FNB + 1; // padded row stride of the 32x32 blocks
+
+    __shared__ TC sbuf[NMAX * SB];
+    __shared__ TC sU[FNB * UB];
+    __shared__ int spi[NMAX]; // logical row -> original row
+    __shared__ short ssig[SMAX * NMAX]; // block t: logical row -> storage row
+    __shared__ TC scand[2 * NW * FNB];
+    __shared__ TC skey[2 * NW];
+    __shared__ int slab[2 * NW];
+
+    const int64_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int w = tid >> 6;
+    const TI* Ai = AI[b];
+    TC* Ao = AO[b];
+    int* ipiv = ipivA + b * strideP;
+    const int S = (n + FNB - 1) / FNB;
+
+    spi[tid] = tid;
+    int zero_at = 0;
+    __syncthreads();
+
+    for(int s = 0; s < S; ++s)
+    {
+        const int c0 = s * FNB;
+        const int jb = (n - c0) < FNB ? (n - c0) : FNB;
+        acc_t acc[4][2];
+
+        // ---- 1. gather block column s into the accumulators (logical order)
+#pragma unroll
+        for(int ct = 0; ct < 2; ++ct)
+        {
+            for(int e = tid; e < FHC * NMAX; e += NMAX)
+            {
+                const int row = e % NMAX, j = e / NMAX;
+                const int col = c0 + FHC * ct + j;
+                sbuf[j * NMAX + row]
+                    = (row < n && col < n) ? TC(Ai[row + int64_t(col) * ldi]) : TC(0);
+            }
+            __syncthreads();
+#pragma unroll
+            for(int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                {
+                    const int i = 64 * w + 16 * rt + M::row(lane, r);
+                    acc[rt][ct][r] = sbuf[(lane & 15) * NMAX + spi[i]];
+                }
+            __syncthreads();
+        }
+
+        // ---- 2. left-looking update with every factored block column t < s
+        for(int t = 0; t < s; ++t)
+        {
+            const int r0 = t * FNB;
+            const int r1 = r0 + FNB;
+            const int wt = r0 >> 6; // wave holding logical rows [r0, r1)
+            const int half = (r0 >> 5) & 1; // which 32-row half of that wave
+            if(w == wt)
+            {
+#pragma unroll
+                for(int rt = 0; rt < 4; ++rt)
+                    if((rt >> 1) == half)
+#pragma unroll
+                        for(int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                            for(int r = 0; r < 4; ++r)
+                            {
+                                const int q = 16 * (rt & 1) + M::row(lane, r);
+                                sU[q * UB + 16 * ct + (lane & 15)] = acc[rt][ct][r];
+                            }
+            }
+            for(int e = tid; e < FNB * FNB; e += NMAX)
+            {
+                const int i = e % FNB, k = e / FNB;
+                sbuf[i * UB + k] = Ao[(r0 + i) + int64_t(r0 + k) * ldo]; // L_tt (final)
+            }
+            __syncthreads();
+            {
+                // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i of
+                // CPT right-hand columns; x_k reaches the group by a shuffle
+                constexpr int CPT = FNB * FNB / NMAX;
+                const int i = tid & (FNB - 1);
+                const int j0 = (tid / FNB) * CPT;
+                TC x[CPT];
+#pragma unroll
+                for(int q = 0; q < CPT; ++q)
+                    x[q] = sU[i * UB + j0 + q];
+#pragma unroll
+                for(int k = 0; k < FNB - 1; ++k)
+                {
+                    const TC lik = sbuf[i * UB + k];
+#pragma unroll
+                    for(int q = 0; q < CPT; ++q)
+                    {
+                        const TC xk = __shfl(x[q], k, FNB);
+                        if(i > k)
+                            x[q] = fma_t(-lik, xk, x[q]);
+                    }
+                }
+#pragma unroll
+                for(int q = 0; q < CPT; ++q)
+                    sU[i * UB + j0 + q] = x[q];
+            }
+            __syncthreads();
+            if(w == wt)
+            {
+#pragma unroll
+                for(int rt = 0; rt < 4; ++rt)
+                    if((rt >> 1) == half)
+#pragma unroll
+                        for(int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                            for(int r = 0; r < 4; ++r)
+                            {
+                                const int q = 16 * (rt & 1) + M::row(lane, r);
+                                acc[rt][ct][r] = sU[q * UB + 16 * ct + (lane & 15)];
+                            }
+            }
+            // C[rows >= r1] -= L21_t * U_ts, 8 columns of L21_t staged at a time
+            for(int k0 = 0; k0 < FNB; k0 += 8)
+            {
+                for(int e = tid; e < 8 * NMAX; e += NMAX)
+                {
+                    const int row = e % NMAX, k = e / NMAX;
+                    sbuf[k * NMAX + row] = (row >= r1 && row < n)
+                                               ? Ao[row + int64_t(r0 + k0 + k) * ldo]
+                                               : TC(0);
+                }
+                __syncthreads();
+#pragma unroll
+                for(int kk = 0; kk < 8; kk += 4)
+                {
+                    TC bop[2];
+#pragma unroll
+                    for(int ct = 0; ct < 2; ++ct)
+                        bop[ct] = sU[(k0 + kk + (lane >> 4)) * UB + 16 * ct + (lane & 15)];
+#pragma unroll
+                    for(int rt = 0; rt < 4; ++rt)
+                    {
+                        if(64 * w + 16 * rt >= r1) // wave-uniform
+                        {
+                            const int i = 64 * w + 16 * rt + (lane & 15);
+                            const TC aop
+                                = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
+#pragma unroll
+                            for(int ct = 0; ct < 2; ++ct)
+                                acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // ---- 3. panel: one logical row per thread
+        TC rowv[FNB];
+#pragma unroll
+        for(int ct = 0; ct < 2; ++ct)
+        {
+#pragma unroll
+            for(int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                for(int r = 0; r < 4; ++r)
+                {
+                    const int i = 64 * w + 16 * rt + M::row(lane, r);
+                    sbuf[i * SB + (lane & 15)] = acc[rt][ct][r];
+                }
+            __syncthreads();
+#pragma unroll
+            for(int j = 0; j < FHC; ++j)
+                rowv[FHC * ct + j] = sbuf[tid * SB + j];
+            __syncthreads();
+        }
+
+        int label = tid; // logical row held by this thread
+#pragma unroll
+        for(int k = 0; k < FNB; ++k)
+        {
+            if(k < jb) // uniform
+            {
+                const int c = c0 + k;
+                const int par = k & 1;
+                TC key = (label >= c && label < n) ? pivot_key(rowv[k], label == c) : TC(-1);
+                int lab = label;
+                wave_argmax(key, lab);
+                if(label == lab)
+                {
+#pragma unroll
+                    for(int j = k; j < FNB; ++j)
+                        scand[(par * NW + w) * FNB + j] = rowv[j];
+                    skey[par * NW + w] = key;
+                    slab[par * NW + w] = lab;
+                }
+                __syncthreads();
+                int wb = 0;
+                TC bk = skey[par * NW];
+                int bl = slab[par * NW];
+#pragma unroll
+                for(int ww = 1; ww < NW; ++ww)
+                {
+                    const TC kw = skey[par * NW + ww];
+                    const int lw = slab[par * NW + ww];
+                    if(kw > bk || (kw == bk && lw < bl))
+                    {
+                        bk = kw;
+                        bl = lw;
+                        wb = ww;
+                    }
+                }
+                const TC* prow = scand + (par * NW + wb) * FNB;
+                const TC pv = prow[k];
+                const bool nz = pv != TC(0);
+                if(!nz && zero_at == 0)
+                    zero_at = c + 1;
+                if(tid == 0)
+                {
+                    ipiv[c] = bl + 1;
+                    if(bl != c)
+                    {
+                        const int tp = spi[c];
+                        spi[c] = spi[bl];
+                        spi[bl] = tp;
+                        for(int tt = 0; tt < s; ++tt)
+                        {
+                            const short q = ssig[tt * NMAX + c];
+                            ssig[tt * NMAX + c] = ssig[tt * NMAX + bl];
+                            ssig[tt * NMAX + bl] = q;
+                        }
+                    }
+                }
+                if(label == bl)
+                    label = c;
+                else if(label == c)
+                    label = bl;
+                if(label > c && label < n)
+                {
+                    TC l = rowv[k];
+                    if(nz)
+                    {
+                        l *= TC(1) / pv;
+                        rowv[k] = l;
+                    }
+#pragma unroll
+                    for(int j = k + 1; j < FNB; ++j)
+                        rowv[j] = fma_t(-l, prow[j], rowv[j]);
+                }
+            }
+        }
+
+        // ---- 4. write block column s in the logical order of this moment
+#pragma unroll
+        for(int j = 0; j < FNB; ++j)
+            if(j < jb && label < n)
+                Ao[label + int64_t(c0 + j) * ldo] = rowv[j];
+        ssig[s * NMAX + tid] = short(tid);
+        __syncthreads();
+    }
+
+    // ---- move every L21_t into the final row order
+    for(int t = 0; t + 1 < S; ++t)
+    {
+        const int r1 = (t + 1) * FNB;
+#pragma unroll
+        for(int ct = 0; ct < 2; ++ct)
+        {
+            const int cb = t * FNB + FHC * ct;
+            for(int e = tid; e < FHC * NMAX; e += NMAX)
+            {
+                const int row = e % NMAX, j = e / NMAX;
+                if(row >= r1 && row < n)
+                    sbuf[j * NMAX + row] = Ao[row + int64_t(cb + j) * ldo];
+            }
+            __syncthreads();
+            for(int e = tid; e < FHC * NMAX; e += NMAX)
+            {
+                const int row = e % NMAX, j = e / NMAX;
+                if(row >= r1 && row < n)
+                    Ao[row + int64_t(cb + j) * ldo] = sbuf[j * NMAX + ssig[t * NMAX + row]];
+            }
+            __syncthreads();
+        }
+    }
+    if(tid == 0)
+        infoA[b] = zero_at;
+}
+
+template <typename TI, typename TC>
+rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO,
+                               int ldo, int* ipiv, int64_t strideP, int* info, int bc)
+{
+    if(bc <= 0 || n <= 0)
+        return rocblas_status_success;
+    if(n <= 128)
+        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 128>), dim3(bc), dim3(128), 0, stream, n, AI,
+                           ldi, AO, ldo, ipiv, strideP, info, bc);
+    else
+        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 256>), dim3(bc), dim3(256), 0, stream, n, AI,
+                           ldi, AO, ldo, ipiv, strideP, info, bc);
+    RSA_LAUNCH_CHECK();
+    return rocblas_status_success;
+}
+
+} // namespace rsa
