@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched LU engine on one MI355X.
+
+Metric: solves/s for n=256, batch 4096, 8 right-hand sides, reaching FP64
+backward error with FP32-MFMA factorisation + FP64 iterative refinement
+(rocsolver_amd_dsgesv_strided_batched), compared with plain FP64 dgesv on the
+same GPU.  A step is one batched solve of the whole batch with A and B
+resident in HBM.  Inputs: A = U diag(s) V^T (U, V orthogonal from QR of
+N(0,1) matrices, s log-spaced in [1, kappa], kappa log-uniform in
+[1e2, 1e6] per matrix), B ~ N(0,1), seeded, generated on the GPU.
+
+Prints one JSON line (rank 0).  Extra fields: plain FP64 dgesv throughput
+(this library, and the image's rocSOLVER when loadable), accuracy of the
+timed solves, the dominant kernel's roofline and the host-LAPACK CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, BC, NRHS = 256, 4096, 8
+FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+EPS64 = 2.0 ** -52
+
+
+def lu_flops(n: int) -> float:
+    """LAPACK getrf operation count (2n^3/3 - n^2/2 - n/6), the reference's
+    GFLOP convention (scripts/perf/postprocess.py:14-18)."""
+    return 2.0 * n ** 3 / 3.0 - n ** 2 / 2.0 - n / 6.0
+
+
+def gesv_flops(n: int, nrhs: int) -> float:
+    return lu_flops(n) + nrhs * (2.0 * n * n - n)
+
+
+def event_time(torch, fn, reps: int, stream=None) -> float:
+    """Mean duration (s) of fn() measured with HIP events on the stream."""
+    s = stream or torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    total = 0.0
+    for _ in range(reps):
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        total += e0.elapsed_time(e1) * 1e-3
+    return total / reps
+
+
+def cpu_baseline(torch, A, B, X_gpu, budget_s: float):
+    """Host LAPACK dgesv_ over a bounded sample of the same batch (the
+    reference clients' CPU path, cpu_gesv -> dgesv_), single thread and all
+    host cores; also the forward error of the GPU solutions on that sample."""
+    import numpy as np
+
+    from oracle import lapack
+    if not lapack.available():
+        return None
+    lapack.set_num_threads(1)
+    sample = min(BC, 1024)
+    Ah = A[:sample].cpu().numpy()
+    Bh = B[:sample].cpu().numpy()
+    Xg = X_gpu[:sample].cpu().numpy()
+    _, _, desc = lapack.load()
+
+    def solve_range(lo, hi, out):
+        ipiv = np.zeros(N, dtype=np.int32)
+        for i in range(lo, hi):
+            a = Ah[i].copy()
+            x = Bh[i].copy()
+            lapack.gesv_one(N, NRHS, a.ctypes.data, N, ipiv.ctypes.data, x.ctypes.data, N)
+            out[i] = x
+
+    Xc = np.zeros_like(Bh)
+    # single thread, until the budget or the sample is spent
+    t0 = time.perf_counter()
+    done = 0
+    while done < sample and time.perf_counter() - t0 < budget_s:
+        hi = min(sample, done + 16)
+        solve_range(done, hi, Xc)
+        done = hi
+    t1 = time.perf_counter()
+    rate1 = done / (t1 - t0)
+    cores = len(os.sched_getaffinity(0))
+    threads = min(cores, 64)
+    chunks = [(i, min(sample, i + 8)) for i in range(0, sample, 8)]
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda c: solve_range(c[0], c[1], Xc), chunks))
+    t1 = time.perf_counter()
+    rate_all = sample / (t1 - t0)
+    fe = np.abs(Xg - Xc).reshape(sample, -1).max(axis=1) / np.abs(Xc).reshape(sample, -1).max(axis=1)
+    return {
+        "value": round(rate_all, 1), "unit": "solves/s", "cores": threads, "kind": "reference",
+        "sample": (f"dgesv_ of {sample} matrices of the same n=256/8-RHS batch through host "
+                   f"LAPACK ({desc}), the LAPACK call the reference clients time (cpu_gesv); "
+                   f"{threads} threads each calling single-threaded LAPACK; one thread: "
+                   f"{rate1:.1f} solves/s over {done} matrices"),
+        "single_thread_value": round(rate1, 1),
+        "gpu_forward_error_max_vs_lapack": float(fe.max()),
+    }
+
+
+def rocsolver_reference_gpu(torch, handle, A, B, reps: int):
+    """Plain FP64 dgesv of the image's rocSOLVER (comparison only)."""
+    path = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "librocsolver.so")
+    try:
+        rs = ctypes.CDLL(path)
+    except OSError:
+        return None
+    P, I, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    f = rs.rocsolver_dgesv_strided_batched
+    f.argtypes = [P, I, I, P, I, S, P, S, P, I, S, P, I]
+    Aw, Bw = A.clone(), B.clone()
+    ipiv = torch.zeros((BC, N), dtype=torch.int32, device="cuda")
+    info = torch.zeros(BC, dtype=torch.int32, device="cuda")
+
+    def run():
+        f(handle.value, N, NRHS, Aw.data_ptr(), N, N * N, ipiv.data_ptr(), N, Bw.data_ptr(), N,
+          N * NRHS, info.data_ptr(), BC)
+    total = 0.0
+    for _ in range(reps):
+        Aw.copy_(A)
+        Bw.copy_(B)
+        torch.cuda.synchronize()
+        total += event_time(torch, run, 1)
+    return BC / (total / reps)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=8.0,
+                    help="seconds of single-thread host LAPACK in the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every matrix is "
+              "independent, so the batch would be split across GPUs with no exchange)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    import rocsolver_amd as ra
+    from rocsolver_amd import workloads as wl
+
+    torch.cuda.set_device(0)
+    A, B, kappa = wl.config3_torch(BC, N, NRHS, device="cuda")
+    torch.cuda.synchronize()
+    handle = ra.Handle()
+    X = torch.zeros((BC, NRHS, N), dtype=torch.float64, device="cuda")
+    it = torch.zeros(BC, dtype=torch.int32, device="cuda")
+    info = torch.zeros(BC, dtype=torch.int32, device="cuda")
+    berr = torch.zeros(BC, dtype=torch.float64, device="cuda")
+
+    def mixed_step():
+        ra.check("rocsolver_amd_dsgesv_strided_batched", handle.value, N, NRHS, A.data_ptr(), N,
+                 N * N, B.data_ptr(), N, N * NRHS, X.data_ptr(), N, N * NRHS, it.data_ptr(),
+                 info.data_ptr(), berr.data_ptr(), BC)
+
+    for _ in range(args.warmup):
+        mixed_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mixed_step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = BC * args.steps / elapsed
+
+    # accuracy of the timed solves, measured on the GPU in FP64
+    Arm = A.transpose(1, 2)  # row-major view of each column-major matrix
+    R = B.transpose(1, 2) - torch.matmul(Arm, X.transpose(1, 2))
+    anorm = Arm.abs().sum(dim=2).amax(dim=1)
+    rn = R.abs().amax(dim=1)
+    xn = X.abs().amax(dim=2)
+    bn = B.abs().amax(dim=2)
+    bwd = (rn / (anorm[:, None] * xn + bn)).amax(dim=1)
+    iters = it.cpu()
+    hist = {int(k): int(v) for k, v in zip(*torch.unique(iters, return_counts=True))}
+
+    # plain FP64 dgesv of this library (A is overwritten: refreshed outside the timing)
+    Aw, Bw = A.clone(), B.clone()
+    ipiv = torch.zeros((BC, N), dtype=torch.int32, device="cuda")
+    info64 = torch.zeros(BC, dtype=torch.int32, device="cuda")
+
+    def plain_step():
+        ra.check("rocsolver_dgesv_strided_batched", handle.value, N, NRHS, Aw.data_ptr(), N, N * N,
+                 ipiv.data_ptr(), N, Bw.data_ptr(), N, N * NRHS, info64.data_ptr(), BC)
+    plain_reps = max(2, min(args.steps, 5))
+    tot = 0.0
+    for r in range(plain_reps + 1):
+        Aw.copy_(A)
+        Bw.copy_(B)
+        torch.cuda.synchronize()
+        t = event_time(torch, plain_step, 1)
+        if r > 0:  # first call is a cold call
+            tot += t
+    plain = BC / (tot / plain_reps)
+    ref_gpu = rocsolver_reference_gpu(torch, handle, A, B, 2)
+    del Aw, Bw
+
+    # dominant kernel: the FP32 factorisation of the FP64 batch, timed alone
+    LU = torch.empty((BC, N, N), dtype=torch.float32, device="cuda")
+    ipiv32 = torch.zeros((BC, N), dtype=torch.int32, device="cuda")
+    info32 = torch.zeros(BC, dtype=torch.int32, device="cuda")
+
+    def lu_step():
+        ra.check("rocsolver_amd_dsgetrf_strided_batched", handle.value, N, A.data_ptr(), N, N * N,
+                 LU.data_ptr(), N, N * N, ipiv32.data_ptr(), N, info32.data_ptr(), BC)
+    lu_step()
+    torch.cuda.synchronize()
+    lu_t = event_time(torch, lu_step, max(3, args.steps))
+    achieved = BC * lu_flops(N) / lu_t / 1e12
+    del LU
+
+    cpu = None if args.no_cpu else cpu_baseline(torch, A, B, X, args.cpu_budget)
+
+    out = {
+        "metric": "solves/s n=256 batch 4096 8 RHS: FP32-MFMA+IR vs plain FP64 dgesv at eps64",
+        "value": round(value, 1),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 LU + f64 refinement",
+        "data": "synthetic (seeded, generated on the GPU)",
+        "config": {"workload": ("mixed-precision batched gesv: n=256, batch 4096, 8 RHS, "
+                                "A = U diag(s) V^T, kappa log-uniform in [1e2, 1e6]"),
+                   "n": N, "batch": BC, "nrhs": NRHS},
+        "plain_fp64_dgesv_solves_per_s": round(plain, 1),
+        "speedup_vs_plain_fp64": round(value / plain, 3),
+        "rocsolver_dgesv_solves_per_s": None if ref_gpu is None else round(ref_gpu, 1),
+        "accuracy": {
+            "max_backward_error": float(bwd.max()),
+            "target_10_n_eps64": 10 * N * EPS64,
+            "iter_histogram": hist,
+            "fallback_matrices": int((iters < 0).sum()),
+            "nonzero_info": int((info != 0).sum()),
+        },
+        "roofline": {
+            "kernel": "lu_fused_kernel<double,float,256> (FP32 LU of the FP64 batch)",
+            "bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_MFMA_PEAK_TFLOPS, 4),
+            "traffic": None, "launch_ms": round(lu_t * 1e3, 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

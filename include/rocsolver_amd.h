@@ -220,6 +220,41 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
                                 const rocblas_int lda, const rocblas_int k1, const rocblas_int k2,
                                 const rocblas_int* ipiv, const rocblas_int incx);
 
+/* ------------------------------------------------- mixed-precision solve
+ * No counterpart in the reference (it has no refinement solver); semantics
+ * of LAPACK DSGESV applied to every matrix of a strided batch:
+ * FP32 LU of A on the matrix cores, FP32 triangular solves, FP64 residual
+ * R = B - A X and update, stopping when every column satisfies
+ * ||R||_inf <= ||X||_inf ||A||_inf sqrt(n) 2^-53, at most 30 corrections;
+ * matrices that do not converge (or whose FP32 copy overflows or is
+ * singular) are solved again with an FP64 LU.
+ *   A, B   device, read only (A is not overwritten, unlike DSGESV's fallback)
+ *   X      device, ldx >= n: the solution
+ *   iter   device, per matrix: >= 0 corrections used; -2 FP32 overflow;
+ *          -3 FP32 LU singular; -31 no convergence (FP64 path used)
+ *   info   device, per matrix: first zero pivot of the FP64 LU on the
+ *          fallback path, else 0 (X is defined only where info == 0)
+ *   berr   device or NULL: per matrix max_c ||R_c||_inf / (||A||_inf ||X_c||_inf)
+ *          at convergence (-1 where the FP64 path produced X)
+ * Workspace comes from the handle (size query supported): about
+ * bc * (8 n^2 + 8 n nrhs) bytes.  Supported orders: 64 < n <= 256
+ * (rocblas_status_not_implemented otherwise). */
+rocblas_status rocsolver_amd_dsgesv_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,
+    const rocblas_int lda, const rocblas_stride strideA, const double* B, const rocblas_int ldb,
+    const rocblas_stride strideB, double* X, const rocblas_int ldx, const rocblas_stride strideX,
+    rocblas_int* iter, rocblas_int* info, double* berr, const rocblas_int batch_count);
+
+/* FP32 LU of FP64 matrices (the factorisation step of the mixed-precision
+ * solve, usable on its own).  No counterpart in the reference.  A (FP64,
+ * read only) is rounded to FP32 and factored with partial pivoting into LU
+ * (FP32, ldlu >= n), with getrf's ipiv/info conventions. */
+rocblas_status rocsolver_amd_dsgetrf_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const double* A, const rocblas_int lda,
+    const rocblas_stride strideA, float* LU, const rocblas_int ldlu, const rocblas_stride strideLU,
+    rocblas_int* ipiv, const rocblas_stride strideP, rocblas_int* info,
+    const rocblas_int batch_count);
+
 /* ---------------------------------------------------------------- version
  * Reference: rocsolver-functions.h:34 and :45; library/src/common/buildinfo.cpp. */
 rocblas_status rocsolver_get_version_string(char* buf, size_t len);

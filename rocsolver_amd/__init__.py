@@ -88,19 +88,31 @@ SIGNATURES = {
 }
 
 
+EXTENSIONS = {
+    "rocsolver_amd_dsgesv_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _I, _S, _P, _I, _S, _P,
+                                             _P, _P, _I],
+    "rocsolver_amd_dsgetrf_strided_batched": [_P, _I, _P, _I, _S, _P, _I, _S, _P, _S, _P, _I],
+}
+
+
 def _declare(L: ctypes.CDLL) -> None:
     for base, args in SIGNATURES.items():
         for pre in ("s", "d"):
             f = getattr(L, f"rocsolver_{pre}{base}")
             f.argtypes = args
             f.restype = ctypes.c_int
+    for name, args in EXTENSIONS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
     L.rocsolver_get_version_string.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     L.rocsolver_get_version_string_size.argtypes = [ctypes.POINTER(ctypes.c_size_t)]
 
 
 def exported_symbols() -> list[str]:
     out = [f"rocsolver_{p}{b}" for b in SIGNATURES for p in ("s", "d")]
-    return out + ["rocsolver_get_version_string", "rocsolver_get_version_string_size"]
+    return out + list(EXTENSIONS) + ["rocsolver_get_version_string",
+                                     "rocsolver_get_version_string_size"]
 
 
 def version() -> str:
@@ -238,3 +250,22 @@ def gesv(handle: Handle, A, B, batched: bool = False):
               n * lda, ipiv.data_ptr(), ipiv.shape[1], B.data_ptr(), ldb, nrhs * ldb,
               info.data_ptr(), bc)
     return ipiv[:, :n], info
+
+
+def dsgesv(handle: Handle, A, B, want_berr: bool = True):
+    """Mixed-precision solve of every system (FP32 LU + FP64 refinement).
+
+    A (bc, n, lda) and B (bc, nrhs, ldb) are float64 and left unchanged.
+    Returns (X (bc, nrhs, n), iter[bc], info[bc], berr[bc] or None)."""
+    import torch
+    bc, n, lda = A.shape
+    _, nrhs, ldb = B.shape
+    dev = A.device
+    X = torch.zeros((bc, nrhs, n), dtype=torch.float64, device=dev)
+    it = torch.zeros(bc, dtype=torch.int32, device=dev)
+    info = torch.zeros(bc, dtype=torch.int32, device=dev)
+    berr = torch.zeros(bc, dtype=torch.float64, device=dev) if want_berr else None
+    check("rocsolver_amd_dsgesv_strided_batched", handle.value, n, nrhs, A.data_ptr(), lda,
+          n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n, nrhs * n, it.data_ptr(),
+          info.data_ptr(), berr.data_ptr() if want_berr else None, bc)
+    return X, it, info, berr

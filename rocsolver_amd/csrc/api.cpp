@@ -90,6 +90,9 @@ rocblas_status getrs_impl(rocblas_handle handle, rocblas_operation trans, int n,
     rocblas_get_stream(handle, &stream);
     if(n <= SMALL_MAX)
         return getrs_small<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true);
+    if(n <= FUSED_MAX)
+        return getrs_fused<T, T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb, bc,
+                                 nullptr, false);
     return getrs_blocked<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
                             nullptr);
 }
@@ -118,9 +121,15 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
         return reset_info(stream, info, bc);
     if(n <= SMALL_MAX)
         return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
-    rocblas_status st = n <= FUSED_MAX
-                            ? lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc)
-                            : getrf_blocked<T>(stream, n, n, A, lda, ipiv, strideP, info, bc, true);
+    if(n <= FUSED_MAX)
+    {
+        rocblas_status st = lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
+        if(st != rocblas_status_success)
+            return st;
+        return getrs_fused<T, T>(stream, false, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb,
+                                 bc, info, false);
+    }
+    rocblas_status st = getrf_blocked<T>(stream, n, n, A, lda, ipiv, strideP, info, bc, true);
     if(st != rocblas_status_success)
         return st;
     return getrs_blocked<T>(stream, false, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
@@ -148,9 +157,96 @@ rocblas_status laswp_impl(rocblas_handle handle, int n, T* A, int lda, int k1, i
     return laswp<T>(stream, n, strided(A, 0), lda, k1, k2, ipiv, 0, incx, 1, nullptr);
 }
 
+constexpr int DSGESV_ITERMAX = 30; // LAPACK dsgesv ITERMAX
+
+rocblas_status dsgesv_impl(rocblas_handle handle, int n, int nrhs, const double* A, int lda,
+                           int64_t strideA, const double* B, int ldb, int64_t strideB, double* X,
+                           int ldx, int64_t strideX, int* iter, int* info, double* berr, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || nrhs < 0 || lda < n || ldb < n || ldx < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    const int64_t nn = int64_t(n) * nrhs;
+    if(!query && ((n && !A) || (nn && (!B || !X)) || (bc && (!iter || !info))))
+        return rocblas_status_invalid_pointer;
+    const bool work_needed = n > 0 && nrhs > 0 && bc > 0;
+    const size_t ws = work_needed ? dsgesv_workspace_bytes(n, nrhs, bc, DSGESV_ITERMAX) : 0;
+    if(query)
+        return rocblas_set_optimal_device_memory_size(handle, ws);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(!work_needed)
+    {
+        rocblas_status st = reset_info(stream, info, bc);
+        return st != rocblas_status_success ? st : reset_info(stream, iter, bc);
+    }
+    if(n <= SMALL_MAX || n > FUSED_MAX)
+        return rocblas_status_not_implemented;
+    rocblas_device_malloc mem(handle, ws);
+    if(!mem)
+        return rocblas_status_memory_error;
+    return dsgesv_refine(stream, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
+                         strided(const_cast<double*>(B), strideB), ldb, strided(X, strideX), ldx,
+                         iter, info, berr, bc, DSGESV_ITERMAX, static_cast<void*>(mem), true);
+}
+
+__global__ void convert_d2s_kernel(int m, int n, Batch<double> A, int lda, Batch<float> O, int ldo)
+{
+    const int64_t b = blockIdx.y;
+    const double* a = A[b];
+    float* o = O[b];
+    for(int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < int64_t(m) * n;
+        e += int64_t(gridDim.x) * blockDim.x)
+    {
+        const int i = int(e % m), j = int(e / m);
+        o[i + int64_t(j) * ldo] = float(a[i + int64_t(j) * lda]);
+    }
+}
+
+rocblas_status dsgetrf_impl(rocblas_handle handle, int n, const double* A, int lda,
+                            int64_t strideA, float* LU, int ldlu, int64_t strideLU, int* ipiv,
+                            int64_t strideP, int* info, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || lda < n || ldlu < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    const int64_t nn = int64_t(n) * n;
+    if(!query && ((nn && (!A || !LU || !ipiv)) || (bc && !info)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(n == 0)
+        return reset_info(stream, info, bc);
+    Batch<double> Ab = strided(const_cast<double*>(A), strideA);
+    Batch<float> Lb = strided(LU, strideLU);
+    if(n > SMALL_MAX && n <= FUSED_MAX)
+        return lu_fused<double, float>(stream, n, Ab, lda, Lb, ldlu, ipiv, strideP, info, bc);
+    // other orders: round to FP32 into LU, then the FP32 factorisation in place
+    for(int64_t b0 = 0; b0 < bc; b0 += 65535)
+    {
+        const int nb = int((bc - b0) < 65535 ? (bc - b0) : 65535);
+        Batch<double> Ai = strided(const_cast<double*>(A) + b0 * strideA, strideA);
+        Batch<float> Li = strided(LU + b0 * strideLU, strideLU);
+        hipLaunchKernelGGL(convert_d2s_kernel, dim3(ceil_div(nn, 256) < 64 ? ceil_div(nn, 256) : 64, nb),
+                           dim3(256), 0, stream, n, n, Ai, lda, Li, ldlu);
+        RSA_LAUNCH_CHECK();
+    }
+    return getrf_impl<float>(handle, n, n, Lb, true, ldlu, ipiv, strideP, info, bc, true);
+}
+
 } // namespace
 
-#define RSA_GETRF_FAMILY(P, T)                                                                  \
+#define RSA_GETRF_FAMILY(P, T)                                                                \
     rocblas_status rocsolver_##P##getrf(rocblas_handle handle, const rocblas_int m,             \
                                         const rocblas_int n, T* A, const rocblas_int lda,       \
                                         rocblas_int* ipiv, rocblas_int* info)                   \
@@ -288,6 +384,26 @@ RSA_GETRF_FAMILY(s, float)
 RSA_GETRF_FAMILY(d, double)
 RSA_GETRS_GESV_FAMILY(s, float)
 RSA_GETRS_GESV_FAMILY(d, double)
+
+rocblas_status rocsolver_amd_dsgesv_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,
+    const rocblas_int lda, const rocblas_stride strideA, const double* B, const rocblas_int ldb,
+    const rocblas_stride strideB, double* X, const rocblas_int ldx, const rocblas_stride strideX,
+    rocblas_int* iter, rocblas_int* info, double* berr, const rocblas_int batch_count)
+{
+    return dsgesv_impl(handle, n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, iter,
+                       info, berr, batch_count);
+}
+
+rocblas_status rocsolver_amd_dsgetrf_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const double* A, const rocblas_int lda,
+    const rocblas_stride strideA, float* LU, const rocblas_int ldlu, const rocblas_stride strideLU,
+    rocblas_int* ipiv, const rocblas_stride strideP, rocblas_int* info,
+    const rocblas_int batch_count)
+{
+    return dsgetrf_impl(handle, n, A, lda, strideA, LU, ldlu, strideLU, ipiv, strideP, info,
+                        batch_count);
+}
 
 #define RSA_VERSION_STRING "3.23.0.gfx950"
 

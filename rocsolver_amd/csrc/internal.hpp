@@ -28,12 +28,22 @@ rocblas_status getrs_small(hipStream_t stream, bool trans, int n, int nrhs, Batc
 // ---------------------------------------------------------- fused LU path
 // Square LU for SMALL_MAX < n <= FUSED_MAX, one workgroup per matrix.  AI is
 // read (type TI), the factors are written to AO (type TC; AO may alias AI
-// when TI == TC).
+// when TI == TC).  Batch entries with skip[b] != 0 are not touched.
 constexpr int FUSED_MAX = 256;
 
 template <typename TI, typename TC>
 rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO, int ldo,
-                        int* ipiv, int64_t strideP, int* info, int bc);
+                        int* ipiv, int64_t strideP, int* info, int bc,
+                        const int* skip = nullptr);
+
+// One-workgroup-per-system solve with the LU of getrf for SMALL_MAX < n <=
+// FUSED_MAX, arithmetic in TL; reads the right-hand sides from B (type TB),
+// writes (or with accum, adds) the solution to X (may alias B).  Entries with
+// skip[b] != 0 are not touched.
+template <typename TL, typename TB>
+rocblas_status getrs_fused(hipStream_t stream, bool trans, int n, int nrhs, Batch<TL> A, int lda,
+                           const int* ipiv, int64_t strideP, Batch<TB> B, int ldb, Batch<TB> X,
+                           int ldx, int bc, const int* skip, bool accum);
 
 // -------------------------------------------------------------- blocked path
 template <typename T>
@@ -46,6 +56,17 @@ template <typename T>
 rocblas_status getrs_blocked(hipStream_t stream, bool trans, int n, int nrhs, Batch<T> A, int lda,
                              const int* ipiv, int64_t strideP, Batch<T> B, int ldb, int bc,
                              bool pivot, const int* skip_info);
+
+// ----------------------------------------------- mixed-precision refinement
+// FP32 LU + FP64 refinement with LAPACK dsgesv semantics per matrix, for
+// SMALL_MAX < n <= FUSED_MAX.  `work` must hold dsgesv_workspace_bytes.
+// early_exit: read one device counter per refinement step and stop once
+// every matrix has converged (otherwise the fixed ITERMAX schedule runs).
+size_t dsgesv_workspace_bytes(int n, int nrhs, int bc, int itermax);
+rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> A, int lda,
+                             Batch<double> B, int ldb, Batch<double> X, int ldx, int* iter,
+                             int* info, double* berr, int bc, int itermax, void* work,
+                             bool early_exit);
 
 // --------------------------------------------------------------- utilities
 rocblas_status reset_info(hipStream_t stream, int* info, int bc);

@@ -41,7 +41,7 @@ template <typename TI, typename TC, int NMAX>
 __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                                         int ldo, int* __restrict__ ipivA,
                                                         int64_t strideP, int* __restrict__ infoA,
-                                                        int bc)
+                                                        int bc, const int* __restrict__ skip)
 {
     using M = mfma16<TC>;
     using acc_t = typename M::acc_t;
@@ -59,6 +59,8 @@ __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, 
     __shared__ int slab[2 * NW];
 
     const int64_t b = blockIdx.x;
+    if(skip && skip[b] != 0)
+        return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
@@ -265,19 +267,22 @@ __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, 
                 if(!nz && zero_at == 0)
                     zero_at = c + 1;
                 if(tid == 0)
-                {
                     ipiv[c] = bl + 1;
-                    if(bl != c)
+                if(bl != c)
+                {
+                    // interchange logical rows c and bl in the row maps: thread
+                    // SMAX keeps spi, thread tt < s the map of block tt
+                    if(tid == SMAX)
                     {
                         const int tp = spi[c];
                         spi[c] = spi[bl];
                         spi[bl] = tp;
-                        for(int tt = 0; tt < s; ++tt)
-                        {
-                            const short q = ssig[tt * NMAX + c];
-                            ssig[tt * NMAX + c] = ssig[tt * NMAX + bl];
-                            ssig[tt * NMAX + bl] = q;
-                        }
+                    }
+                    else if(tid < s)
+                    {
+                        const short q = ssig[tid * NMAX + c];
+                        ssig[tid * NMAX + c] = ssig[tid * NMAX + bl];
+                        ssig[tid * NMAX + bl] = q;
                     }
                 }
                 if(label == bl)
@@ -338,16 +343,17 @@ __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, 
 
 template <typename TI, typename TC>
 rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO,
-                               int ldo, int* ipiv, int64_t strideP, int* info, int bc)
+                               int ldo, int* ipiv, int64_t strideP, int* info, int bc,
+                               const int* skip)
 {
     if(bc <= 0 || n <= 0)
         return rocblas_status_success;
     if(n <= 128)
         hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 128>), dim3(bc), dim3(128), 0, stream, n, AI,
-                           ldi, AO, ldo, ipiv, strideP, info, bc);
+                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
     else
         hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 256>), dim3(bc), dim3(256), 0, stream, n, AI,
-                           ldi, AO, ldo, ipiv, strideP, info, bc);
+                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
     RSA_LAUNCH_CHECK();
     return rocblas_status_success;
 }

@@ -6,8 +6,10 @@ namespace rsa
 {
 template <>
 rocblas_status lu_fused<double, double>(hipStream_t stream, int n, Batch<double> AI, int ldi, Batch<double> AO,
-                                int ldo, int* ipiv, int64_t strideP, int* info, int bc)
+                                int ldo, int* ipiv, int64_t strideP, int* info, int bc,
+                                const int* skip)
 {
-    return lu_fused_launch<double, double>(stream, n, AI, ldi, AO, ldo, ipiv, strideP, info, bc);
+    return lu_fused_launch<double, double>(stream, n, AI, ldi, AO, ldo, ipiv, strideP, info, bc,
+                                             skip);
 }
 } // namespace rsa

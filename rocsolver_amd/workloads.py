@@ -154,14 +154,21 @@ def config3_torch(bc: int = 4096, n: int = 256, nrhs: int = 8, seed: int = BASE_
     kappa = torch.exp(torch.empty(bc, dtype=torch.float64, device=device).uniform_(
         np.log(1e2), np.log(1e6), generator=gen))
     out = torch.empty((bc, n, n), dtype=torch.float64, device=device)
-    chunk = 512
+    chunk = 64  # torch's batched QR fails to allocate its workspace for large batches
     t = torch.linspace(0.0, 1.0, n, dtype=torch.float64, device=device)
+
+    def qr_q(G):
+        try:
+            return torch.linalg.qr(G)[0]
+        except RuntimeError:
+            return torch.linalg.qr(G.cpu())[0].to(device)
+
     for b0 in range(0, bc, chunk):
         b1 = min(bc, b0 + chunk)
         G1 = torch.randn((b1 - b0, n, n), dtype=torch.float64, device=device, generator=gen)
         G2 = torch.randn((b1 - b0, n, n), dtype=torch.float64, device=device, generator=gen)
-        U, _ = torch.linalg.qr(G1)
-        V, _ = torch.linalg.qr(G2)
+        U = qr_q(G1)
+        V = qr_q(G2)
         s = torch.exp(torch.log(kappa[b0:b1])[:, None] * t[None, :]).flip(1)
         M = torch.matmul(U * s[:, None, :], V.transpose(1, 2).contiguous())
         out[b0:b1] = M.transpose(1, 2)

@@ -1,0 +1,72 @@
+"""GPU checks of the mixed-precision solve (rocsolver_amd_dsgesv_strided_batched).
+
+Pinned against the oracle's LAPACK-dsgesv restatement (itself pinned against
+LAPACK DSGESV in test_oracle.py): the same class of outcome per matrix
+(converged by refinement vs FP64 fallback), FP64-level backward error
+||b - A x|| / (||A|| ||x|| + ||b||) <= 10 n eps64 (the target accuracy),
+and forward error against the FP64 solution within kappa * n * eps64.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rocsolver_amd as ra
+from rocsolver_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+EPS64 = np.finfo(np.float64).eps
+
+
+def dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("n,nrhs,bc", [(96, 16, 6), (128, 4, 5), (256, 8, 6), (200, 1, 4),
+                                       (70, 3, 3)])
+def test_dsgesv_against_oracle(torch_gpu, handle, n, nrhs, bc):
+    print(f"dsgesv n={n} nrhs={nrhs} bc={bc}", flush=True)
+    A, B, kappa = wl.config3(bc=bc, n=n, nrhs=nrhs)
+    Xo, it_o, info_o = oracle.dsgesv(A, n, B)
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    X, it, info, berr = ra.dsgesv(handle, Ad, Bd)
+    torch_gpu.cuda.synchronize()
+    X, it, info, berr = X.cpu().numpy(), it.cpu().numpy(), info.cpu().numpy(), berr.cpu().numpy()
+    np.testing.assert_array_equal(Ad.cpu().numpy(), A)  # A untouched
+    np.testing.assert_array_equal(Bd.cpu().numpy(), B)
+    np.testing.assert_array_equal(info, info_o)
+    assert np.all((it >= 0) == (it_o >= 0))  # same outcome class per matrix
+    assert np.all(np.abs(it[it >= 0] - it_o[it >= 0]) <= 2)
+    be = oracle.solve_backward_error(A, n, X, B)
+    assert be.max() <= 10 * n * EPS64, be
+    for b in range(bc):
+        fe = np.abs(X[b] - Xo[b]).max() / np.abs(Xo[b]).max()
+        assert fe <= kappa[b] * n * EPS64 * 10
+        if it[b] >= 0:
+            assert 0 <= berr[b] <= np.sqrt(n) * EPS64 * 1.0001
+
+
+def test_dsgesv_fallback_paths(torch_gpu, handle):
+    g = np.random.default_rng(7)
+    n, nrhs, bc = 80, 2, 4
+    A = g.standard_normal((bc, n, n)) + 6 * np.eye(n)[None]
+    # entry 1: kappa ~ 1e12 -> no convergence -> FP64 path (ITER = -31)
+    U, _ = np.linalg.qr(g.standard_normal((n, n)))
+    V, _ = np.linalg.qr(g.standard_normal((n, n)))
+    A[1] = ((U * np.logspace(0, -12, n)) @ V.T).T
+    # entry 2: singular (zero column) -> FP32 LU zero pivot (-3), FP64 info > 0
+    A[2][10, :] = 0.0
+    # entry 3: an entry beyond the FP32 range -> ITER = -2
+    A[3][0, 0] = 1e39
+    B = g.standard_normal((bc, nrhs, n))
+    Xo, it_o, info_o = oracle.dsgesv(A, n, B)
+    X, it, info, berr = ra.dsgesv(handle, dev(torch_gpu, A), dev(torch_gpu, B))
+    torch_gpu.cuda.synchronize()
+    it, info, X = it.cpu().numpy(), info.cpu().numpy(), X.cpu().numpy()
+    np.testing.assert_array_equal(it[1:], [-31, -3, -2])
+    np.testing.assert_array_equal(it_o[1:], [-31, -3, -2])
+    assert it[0] >= 0
+    np.testing.assert_array_equal(info, info_o)
+    assert info[2] == 11
+    ok = info == 0
+    be = oracle.solve_backward_error(A[ok], n, X[ok], B[ok])
+    assert be.max() <= 10 * n * EPS64
