@@ -255,6 +255,30 @@ rocblas_status rocsolver_amd_dsgetrf_strided_batched(
     rocblas_int* ipiv, const rocblas_stride strideP, rocblas_int* info,
     const rocblas_int batch_count);
 
+/* ---------------------------------------------------------------- adjoint
+ * Reverse-mode derivative of X = A^-1 B for every system of a strided batch.
+ * No counterpart in the reference.  Inputs: the LU and ipiv of A (from getrf
+ * or gesv), the solution X and the upstream gradient dX.  Outputs:
+ *   dB = A^-T dX   (transposed getrs with the stored LU and ipiv; dB may alias dX)
+ *   dA = -dB X^T
+ * All arrays device, column-major; no workspace. */
+rocblas_status rocsolver_amd_sgesv_adjoint_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const float* LU,
+    const rocblas_int lda, const rocblas_stride strideA, const rocblas_int* ipiv,
+    const rocblas_stride strideP, const float* X, const rocblas_int ldx,
+    const rocblas_stride strideX, const float* dX, const rocblas_int lddx,
+    const rocblas_stride strideDX, float* dB, const rocblas_int lddb,
+    const rocblas_stride strideDB, float* dA, const rocblas_int ldda,
+    const rocblas_stride strideDA, const rocblas_int batch_count);
+rocblas_status rocsolver_amd_dgesv_adjoint_strided_batched(
+    rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* LU,
+    const rocblas_int lda, const rocblas_stride strideA, const rocblas_int* ipiv,
+    const rocblas_stride strideP, const double* X, const rocblas_int ldx,
+    const rocblas_stride strideX, const double* dX, const rocblas_int lddx,
+    const rocblas_stride strideDX, double* dB, const rocblas_int lddb,
+    const rocblas_stride strideDB, double* dA, const rocblas_int ldda,
+    const rocblas_stride strideDA, const rocblas_int batch_count);
+
 /* ---------------------------------------------------------------- version
  * Reference: rocsolver-functions.h:34 and :45; library/src/common/buildinfo.cpp. */
 rocblas_status rocsolver_get_version_string(char* buf, size_t len);

@@ -92,6 +92,10 @@ EXTENSIONS = {
     "rocsolver_amd_dsgesv_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _I, _S, _P, _I, _S, _P,
                                              _P, _P, _I],
     "rocsolver_amd_dsgetrf_strided_batched": [_P, _I, _P, _I, _S, _P, _I, _S, _P, _S, _P, _I],
+    "rocsolver_amd_sgesv_adjoint_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S,
+                                                    _P, _I, _S, _P, _I, _S, _P, _I, _S, _I],
+    "rocsolver_amd_dgesv_adjoint_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S,
+                                                    _P, _I, _S, _P, _I, _S, _P, _I, _S, _I],
 }
 
 

@@ -244,6 +244,33 @@ rocblas_status dsgetrf_impl(rocblas_handle handle, int n, const double* A, int l
     return getrf_impl<float>(handle, n, n, Lb, true, ldlu, ipiv, strideP, info, bc, true);
 }
 
+template <typename T>
+rocblas_status adjoint_impl(rocblas_handle handle, int n, int nrhs, const T* LU, int lda,
+                            int64_t strideA, const int* ipiv, int64_t strideP, const T* X,
+                            int ldx, int64_t strideX, const T* dX, int lddx, int64_t strideDX,
+                            T* dB, int lddb, int64_t strideDB, T* dA, int ldda, int64_t strideDA,
+                            int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || nrhs < 0 || lda < n || ldx < n || lddx < n || lddb < n || ldda < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    const int64_t nn = int64_t(n) * nrhs;
+    if(!query && ((n && (!LU || !ipiv || !dA)) || (nn && (!X || !dX || !dB))))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+    if(n == 0 || bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    auto cb = [](const T* p, int64_t s) { return strided(const_cast<T*>(p), s); };
+    return gesv_adjoint<T>(stream, n, nrhs, cb(LU, strideA), lda, ipiv, strideP, cb(X, strideX),
+                           ldx, cb(dX, strideDX), lddx, strided(dB, strideDB), lddb,
+                           strided(dA, strideDA), ldda, bc);
+}
+
 } // namespace
 
 #define RSA_GETRF_FAMILY(P, T)                                                                \
@@ -404,6 +431,23 @@ rocblas_status rocsolver_amd_dsgetrf_strided_batched(
     return dsgetrf_impl(handle, n, A, lda, strideA, LU, ldlu, strideLU, ipiv, strideP, info,
                         batch_count);
 }
+
+#define RSA_ADJOINT(P, T)                                                                       \
+    rocblas_status rocsolver_amd_##P##gesv_adjoint_strided_batched(                             \
+        rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const T* LU,        \
+        const rocblas_int lda, const rocblas_stride strideA, const rocblas_int* ipiv,           \
+        const rocblas_stride strideP, const T* X, const rocblas_int ldx,                        \
+        const rocblas_stride strideX, const T* dX, const rocblas_int lddx,                      \
+        const rocblas_stride strideDX, T* dB, const rocblas_int lddb,                           \
+        const rocblas_stride strideDB, T* dA, const rocblas_int ldda,                           \
+        const rocblas_stride strideDA, const rocblas_int batch_count)                           \
+    {                                                                                           \
+        return adjoint_impl<T>(handle, n, nrhs, LU, lda, strideA, ipiv, strideP, X, ldx,        \
+                               strideX, dX, lddx, strideDX, dB, lddb, strideDB, dA, ldda,       \
+                               strideDA, batch_count);                                          \
+    }
+RSA_ADJOINT(s, float)
+RSA_ADJOINT(d, double)
 
 #define RSA_VERSION_STRING "3.23.0.gfx950"
 

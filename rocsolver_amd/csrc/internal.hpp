@@ -68,6 +68,13 @@ rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> 
                              int* info, double* berr, int bc, int itermax, void* work,
                              bool early_exit);
 
+// ------------------------------------------------------------------ adjoint
+// dB = A^-T dX (transposed solve with the LU and ipiv of A), dA = -dB X^T.
+template <typename T>
+rocblas_status gesv_adjoint(hipStream_t stream, int n, int nrhs, Batch<T> LU, int lda,
+                            const int* ipiv, int64_t strideP, Batch<T> X, int ldx, Batch<T> dX,
+                            int lddx, Batch<T> dB, int lddb, Batch<T> dA, int ldda, int bc);
+
 // --------------------------------------------------------------- utilities
 rocblas_status reset_info(hipStream_t stream, int* info, int bc);
 

@@ -7,8 +7,10 @@
 //      order (coalesced, staged through LDS) and placed in MFMA accumulators
 //      in the current logical row order;
 //   2. left-looking update, for every factored block column t < s:
-//        U_ts = L_tt^-1 C[t-rows]                 (substitution, 32 threads)
-//        C[rows below t] -= L21_t * U_ts          (MFMA 16x16x4, K staged in LDS)
+//        U_ts = L_tt^-1 C[t-rows]        (substitution, half-wave shuffles)
+//        C[rows below t] -= L21_t * U_ts (MFMA 16x16x4, L21_t staged in LDS in
+//                                         16-column stages prefetched one stage
+//                                         ahead into registers)
 //   3. panel: the accumulators are transposed through LDS to one logical row
 //      per thread and factored with partial pivoting, one barrier per column:
 //      every wave publishes its best candidate row (arg-max, ties to the
@@ -34,11 +36,12 @@ namespace rsa
 constexpr int FNB = 32; // block-column width
 constexpr int FHC = 16; // columns per LDS round of the staging/transposition buffer
 
-// At least 3 waves per SIMD: without the bound the compiler spreads into
-// ~300 registers (1 wave per SIMD); with it the kernel fits ~150 VGPRs and
-// no scratch, so 3 workgroups of 256 threads share a CU.
+// At least 2 waves per SIMD: without the bound the compiler spreads into
+// 300-400 registers (1 wave per SIMD); with it the kernel fits 190-240 VGPRs
+// without scratch (a bound of 3 spills), so 2 workgroups of 256 threads
+// share a CU.
 template <typename TI, typename TC, int NMAX>
-__global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
+__global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                                         int ldo, int* __restrict__ ipivA,
                                                         int64_t strideP, int* __restrict__ infoA,
                                                         int bc, const int* __restrict__ skip)
@@ -49,9 +52,16 @@ __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, 
     constexpr int SMAX = NMAX / FNB;
     constexpr int SB = FHC + 1; // padded row stride of the row-major view
     constexpr int UB = FNB + 1; // padded row stride of the 32x32 blocks
+    // columns of a block column staged per LDS round (half a block: keeps the
+    // prefetch registers and LDS -- ~32 KB FP32, ~58 KB FP64 -- small)
+    constexpr int KC = FNB / 2;
+    constexpr int NKC = FNB / KC;
+    constexpr int SBUF = (NMAX * SB > KC * NMAX) ? NMAX * SB : KC * NMAX;
+    constexpr int LTT = FNB * FNB / NMAX; // L_tt elements staged per thread
 
-    __shared__ TC sbuf[NMAX * SB];
+    __shared__ TC sbuf[SBUF]; // gather / L21 staging [col][row], transposition [row][SB]
     __shared__ TC sU[FNB * UB];
+    __shared__ TC sLtt[FNB * UB];
     __shared__ int spi[NMAX]; // logical row -> original row
     __shared__ short ssig[SMAX * NMAX]; // block t: logical row -> storage row
     __shared__ TC scand[2 * NW * FNB];
@@ -79,123 +89,155 @@ __global__ __launch_bounds__(NMAX, 3) void lu_fused_kernel(int n, Batch<TI> AI, 
         const int jb = (n - c0) < FNB ? (n - c0) : FNB;
         acc_t acc[4][2];
 
-        // ---- 1. gather block column s into the accumulators (logical order)
+        // ---- 1. gather block column s into the accumulators (logical order),
+        //         KC columns per round: thread tid loads original row tid
 #pragma unroll
-        for(int ct = 0; ct < 2; ++ct)
+        for(int kc = 0; kc < NKC; ++kc)
         {
-            for(int e = tid; e < FHC * NMAX; e += NMAX)
+#pragma unroll
+            for(int j = 0; j < KC; ++j)
             {
-                const int row = e % NMAX, j = e / NMAX;
-                const int col = c0 + FHC * ct + j;
-                sbuf[j * NMAX + row]
-                    = (row < n && col < n) ? TC(Ai[row + int64_t(col) * ldi]) : TC(0);
+                const int col = c0 + KC * kc + j;
+                sbuf[j * NMAX + tid]
+                    = (tid < n && col < n) ? TC(Ai[tid + int64_t(col) * ldi]) : TC(0);
             }
             __syncthreads();
 #pragma unroll
-            for(int rt = 0; rt < 4; ++rt)
+            for(int ct = 0; ct < 2; ++ct)
+                if(ct / (2 / NKC) == kc)
 #pragma unroll
-                for(int r = 0; r < 4; ++r)
-                {
-                    const int i = 64 * w + 16 * rt + M::row(lane, r);
-                    acc[rt][ct][r] = sbuf[(lane & 15) * NMAX + spi[i]];
-                }
+                    for(int rt = 0; rt < 4; ++rt)
+#pragma unroll
+                        for(int r = 0; r < 4; ++r)
+                        {
+                            const int i = 64 * w + 16 * rt + M::row(lane, r);
+                            const int j = 16 * ct - KC * kc + (lane & 15);
+                            acc[rt][ct][r] = sbuf[j * NMAX + spi[i]];
+                        }
             __syncthreads();
         }
 
-        // ---- 2. left-looking update with every factored block column t < s
+        // ---- 2. left-looking update with every factored block column t < s.
+        // Stage (t, kc) = columns [KC*kc, KC*kc+KC) of L21_t (and L_tt when
+        // kc == 0).  Its global loads are issued into registers one stage
+        // ahead, right after the previous stage's first barrier, so they are
+        // in flight during that stage's substitution and MFMAs.
+        TC pre[KC];
+        TC preL[LTT];
+        auto issue = [&](int t, int kc) {
+            const int r0 = t * FNB, r1 = r0 + FNB;
+#pragma unroll
+            for(int j = 0; j < KC; ++j)
+                pre[j] = (tid >= r1 && tid < n) ? Ao[tid + int64_t(r0 + KC * kc + j) * ldo] : TC(0);
+            if(kc == 0)
+#pragma unroll
+                for(int e = 0; e < LTT; ++e)
+                {
+                    const int idx = tid + e * NMAX, i = idx % FNB, k = idx / FNB;
+                    preL[e] = Ao[(r0 + i) + int64_t(r0 + k) * ldo]; // L_tt (final)
+                }
+        };
+        if(s > 0)
+            issue(0, 0);
         for(int t = 0; t < s; ++t)
         {
             const int r0 = t * FNB;
             const int r1 = r0 + FNB;
             const int wt = r0 >> 6; // wave holding logical rows [r0, r1)
             const int half = (r0 >> 5) & 1; // which 32-row half of that wave
-            if(w == wt)
+#pragma unroll
+            for(int kc = 0; kc < NKC; ++kc)
             {
 #pragma unroll
-                for(int rt = 0; rt < 4; ++rt)
-                    if((rt >> 1) == half)
-#pragma unroll
-                        for(int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                            for(int r = 0; r < 4; ++r)
-                            {
-                                const int q = 16 * (rt & 1) + M::row(lane, r);
-                                sU[q * UB + 16 * ct + (lane & 15)] = acc[rt][ct][r];
-                            }
-            }
-            for(int e = tid; e < FNB * FNB; e += NMAX)
-            {
-                const int i = e % FNB, k = e / FNB;
-                sbuf[i * UB + k] = Ao[(r0 + i) + int64_t(r0 + k) * ldo]; // L_tt (final)
-            }
-            __syncthreads();
-            {
-                // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i of
-                // CPT right-hand columns; x_k reaches the group by a shuffle
-                constexpr int CPT = FNB * FNB / NMAX;
-                const int i = tid & (FNB - 1);
-                const int j0 = (tid / FNB) * CPT;
-                TC x[CPT];
-#pragma unroll
-                for(int q = 0; q < CPT; ++q)
-                    x[q] = sU[i * UB + j0 + q];
-#pragma unroll
-                for(int k = 0; k < FNB - 1; ++k)
+                for(int j = 0; j < KC; ++j)
+                    sbuf[j * NMAX + tid] = pre[j];
+                if(kc == 0)
                 {
-                    const TC lik = sbuf[i * UB + k];
 #pragma unroll
-                    for(int q = 0; q < CPT; ++q)
+                    for(int e = 0; e < LTT; ++e)
                     {
-                        const TC xk = __shfl(x[q], k, FNB);
-                        if(i > k)
-                            x[q] = fma_t(-lik, xk, x[q]);
+                        const int idx = tid + e * NMAX, i = idx % FNB, k = idx / FNB;
+                        sLtt[i * UB + k] = preL[e];
+                    }
+                    if(w == wt)
+                    {
+#pragma unroll
+                        for(int rt = 0; rt < 4; ++rt)
+                            if((rt >> 1) == half)
+#pragma unroll
+                                for(int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                                    for(int r = 0; r < 4; ++r)
+                                    {
+                                        const int q = 16 * (rt & 1) + M::row(lane, r);
+                                        sU[q * UB + 16 * ct + (lane & 15)] = acc[rt][ct][r];
+                                    }
                     }
                 }
-#pragma unroll
-                for(int q = 0; q < CPT; ++q)
-                    sU[i * UB + j0 + q] = x[q];
-            }
-            __syncthreads();
-            if(w == wt)
-            {
-#pragma unroll
-                for(int rt = 0; rt < 4; ++rt)
-                    if((rt >> 1) == half)
-#pragma unroll
-                        for(int ct = 0; ct < 2; ++ct)
-#pragma unroll
-                            for(int r = 0; r < 4; ++r)
-                            {
-                                const int q = 16 * (rt & 1) + M::row(lane, r);
-                                acc[rt][ct][r] = sU[q * UB + 16 * ct + (lane & 15)];
-                            }
-            }
-            // C[rows >= r1] -= L21_t * U_ts, 8 columns of L21_t staged at a time
-            for(int k0 = 0; k0 < FNB; k0 += 8)
-            {
-                for(int e = tid; e < 8 * NMAX; e += NMAX)
-                {
-                    const int row = e % NMAX, k = e / NMAX;
-                    sbuf[k * NMAX + row] = (row >= r1 && row < n)
-                                               ? Ao[row + int64_t(r0 + k0 + k) * ldo]
-                                               : TC(0);
-                }
                 __syncthreads();
+                {
+                    const int nkc = (kc + 1 == NKC) ? 0 : kc + 1;
+                    const int nt = (kc + 1 == NKC) ? t + 1 : t;
+                    if(nt < s)
+                        issue(nt, nkc);
+                }
+                if(kc == 0)
+                {
+                    // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i
+                    // of CPT right-hand columns; x_k reaches the group by a shuffle
+                    constexpr int CPT = FNB * FNB / NMAX;
+                    const int i = tid & (FNB - 1);
+                    const int j0 = (tid / FNB) * CPT;
+                    TC x[CPT];
 #pragma unroll
-                for(int kk = 0; kk < 8; kk += 4)
+                    for(int q = 0; q < CPT; ++q)
+                        x[q] = sU[i * UB + j0 + q];
+#pragma unroll
+                    for(int k = 0; k < FNB - 1; ++k)
+                    {
+                        const TC lik = sLtt[i * UB + k];
+#pragma unroll
+                        for(int q = 0; q < CPT; ++q)
+                        {
+                            const TC xk = __shfl(x[q], k, FNB);
+                            if(i > k)
+                                x[q] = fma_t(-lik, xk, x[q]);
+                        }
+                    }
+#pragma unroll
+                    for(int q = 0; q < CPT; ++q)
+                        sU[i * UB + j0 + q] = x[q];
+                    __syncthreads();
+                    if(w == wt)
+                    {
+#pragma unroll
+                        for(int rt = 0; rt < 4; ++rt)
+                            if((rt >> 1) == half)
+#pragma unroll
+                                for(int ct = 0; ct < 2; ++ct)
+#pragma unroll
+                                    for(int r = 0; r < 4; ++r)
+                                    {
+                                        const int q = 16 * (rt & 1) + M::row(lane, r);
+                                        acc[rt][ct][r] = sU[q * UB + 16 * ct + (lane & 15)];
+                                    }
+                    }
+                }
+                // C[rows >= r1] -= L21_t[:, KC*kc : KC*kc+KC] * U_ts[KC*kc : ..., :]
+#pragma unroll 2
+                for(int kk = 0; kk < KC; kk += 4)
                 {
                     TC bop[2];
 #pragma unroll
                     for(int ct = 0; ct < 2; ++ct)
-                        bop[ct] = sU[(k0 + kk + (lane >> 4)) * UB + 16 * ct + (lane & 15)];
+                        bop[ct] = sU[(KC * kc + kk + (lane >> 4)) * UB + 16 * ct + (lane & 15)];
 #pragma unroll
                     for(int rt = 0; rt < 4; ++rt)
                     {
                         if(64 * w + 16 * rt >= r1) // wave-uniform
                         {
                             const int i = 64 * w + 16 * rt + (lane & 15);
-                            const TC aop
-                                = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
+                            const TC aop = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
 #pragma unroll
                             for(int ct = 0; ct < 2; ++ct)
                                 acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
