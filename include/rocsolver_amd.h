@@ -237,7 +237,7 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
  *   berr   device or NULL: per matrix max_c ||R_c||_inf / (||A||_inf ||X_c||_inf)
  *          at convergence (-1 where the FP64 path produced X)
  * Workspace comes from the handle (size query supported): about
- * bc * (8 n^2 + 8 n nrhs) bytes.  Supported orders: 64 < n <= 256
+ * bc * (8 n^2 + 8 n nrhs) bytes.  Supported orders: 64 < n <= 512
  * (rocblas_status_not_implemented otherwise). */
 rocblas_status rocsolver_amd_dsgesv_strided_batched(
     rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,

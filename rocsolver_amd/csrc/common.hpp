@@ -74,22 +74,121 @@ __device__ __forceinline__ T pivot_key(T a, bool is_first_row)
     return abs_t(a);
 }
 
-// Arg-max of (key, index) over the whole wavefront, ties to the smaller
-// index; every lane ends with the winner.
+// ----------------------------------------------------------------------------
+// Cross-lane reductions over groups of TS = 16, 32 or 64 lanes without LDS:
+// DPP quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and row_mirror reduce
+// within each 16-lane row; v_permlane16_swap pairs rows (0,1) and (2,3) and
+// v_permlane32_swap the two halves of the wave.  With both operands equal to
+// x, the two results of a swap hold x and the partner row's x in every lane,
+// so max/min of the pair is the pairwise reduction.
+template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+struct lane_pair
+{
+    int a, b;
+};
+__device__ __forceinline__ lane_pair swap16(int v)
+{
+    auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return {int(p[0]), int(p[1])};
+}
+__device__ __forceinline__ lane_pair swap32(int v)
+{
+    auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return {int(p[0]), int(p[1])};
+}
+
+// maximum of a float/double over the group (every lane gets it)
+template <int TS>
+__device__ __forceinline__ float group_max(float v)
+{
+    v = fmaxf(v, __int_as_float(dpp_mov<0xB1>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_mov<0x4E>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_mov<0x141>(__float_as_int(v))));
+    v = fmaxf(v, __int_as_float(dpp_mov<0x140>(__float_as_int(v))));
+    if(TS >= 32)
+    {
+        lane_pair p = swap16(__float_as_int(v));
+        v = fmaxf(__int_as_float(p.a), __int_as_float(p.b));
+    }
+    if(TS >= 64)
+    {
+        lane_pair p = swap32(__float_as_int(v));
+        v = fmaxf(__int_as_float(p.a), __int_as_float(p.b));
+    }
+    return v;
+}
+
+__device__ __forceinline__ double dbl_from(int lo, int hi)
+{
+    return __hiloint2double(hi, lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v)
+{
+    return dbl_from(dpp_mov<CTRL>(__double2loint(v)), dpp_mov<CTRL>(__double2hiint(v)));
+}
+
+template <int TS>
+__device__ __forceinline__ double group_max(double v)
+{
+    v = fmax(v, dpp_mov_d<0xB1>(v));
+    v = fmax(v, dpp_mov_d<0x4E>(v));
+    v = fmax(v, dpp_mov_d<0x141>(v));
+    v = fmax(v, dpp_mov_d<0x140>(v));
+    if(TS >= 32)
+    {
+        lane_pair lo = swap16(__double2loint(v)), hi = swap16(__double2hiint(v));
+        v = fmax(dbl_from(lo.a, hi.a), dbl_from(lo.b, hi.b));
+    }
+    if(TS >= 64)
+    {
+        lane_pair lo = swap32(__double2loint(v)), hi = swap32(__double2hiint(v));
+        v = fmax(dbl_from(lo.a, hi.a), dbl_from(lo.b, hi.b));
+    }
+    return v;
+}
+
+template <int TS>
+__device__ __forceinline__ int group_min(int v)
+{
+    v = min(v, dpp_mov<0xB1>(v));
+    v = min(v, dpp_mov<0x4E>(v));
+    v = min(v, dpp_mov<0x141>(v));
+    v = min(v, dpp_mov<0x140>(v));
+    if(TS >= 32)
+    {
+        lane_pair p = swap16(v);
+        v = min(p.a, p.b);
+    }
+    if(TS >= 64)
+    {
+        lane_pair p = swap32(v);
+        v = min(p.a, p.b);
+    }
+    return v;
+}
+
+// Arg-max of (key, index) over a group of TS lanes, ties to the smaller
+// index; every lane of the group ends with the winner.  Keys must not be
+// NaN (pivot_key maps NaN to +-inf / -1).
+template <int TS, typename T>
+__device__ __forceinline__ void group_argmax(T& key, int& idx)
+{
+    const T m = group_max<TS>(key);
+    idx = group_min<TS>(key == m ? idx : 0x7fffffff);
+    key = m;
+}
+
 template <typename T>
 __device__ __forceinline__ void wave_argmax(T& key, int& idx)
 {
-#pragma unroll
-    for(int off = 1; off < 64; off <<= 1)
-    {
-        T ok = __shfl_xor(key, off, 64);
-        int oi = __shfl_xor(idx, off, 64);
-        if(ok > key || (ok == key && oi < idx))
-        {
-            key = ok;
-            idx = oi;
-        }
-    }
+    group_argmax<64>(key, idx);
 }
 
 inline int ceil_div(int64_t a, int64_t b)

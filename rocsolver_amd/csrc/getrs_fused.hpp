@@ -1,5 +1,5 @@
 // getrs_fused.hpp -- one-workgroup-per-system triangular solves with the LU
-// of getrf, for SMALL_MAX < n <= 256 (reference order of operations:
+// of getrf, for SMALL_MAX < n <= FUSED_MAX (reference order of operations:
 // library/src/lapack/roclapack_getrs.hpp:117-150 -- laswp, then L, then U for
 // op = N; U^T, L^T, then the reverse interchanges for op = T).
 //
@@ -208,29 +208,20 @@ rocblas_status getrs_fused_launch(hipStream_t stream, bool trans, int n, int nrh
         hipLaunchKernelGGL((getrs_fused_kernel<TL, TB, NMAX, NR, ACCUM>), dim3(bc), dim3(NMAX), 0,
                            stream, trans, n, nrhs, A, lda, ipiv, strideP, B, ldb, X, ldx, skip);
     };
-    using I128 = std::integral_constant<int, 128>;
-    using I256 = std::integral_constant<int, 256>;
-    using N1 = std::integral_constant<int, 1>;
-    using N4 = std::integral_constant<int, 4>;
-    using N8 = std::integral_constant<int, 8>;
+    auto by_nrhs = [&](auto nmax) {
+        if(nrhs <= 1)
+            go(nmax, std::integral_constant<int, 1>{});
+        else if(nrhs <= 4)
+            go(nmax, std::integral_constant<int, 4>{});
+        else
+            go(nmax, std::integral_constant<int, 8>{});
+    };
     if(n <= 128)
-    {
-        if(nrhs <= 1)
-            go(I128{}, N1{});
-        else if(nrhs <= 4)
-            go(I128{}, N4{});
-        else
-            go(I128{}, N8{});
-    }
+        by_nrhs(std::integral_constant<int, 128>{});
+    else if(n <= 256)
+        by_nrhs(std::integral_constant<int, 256>{});
     else
-    {
-        if(nrhs <= 1)
-            go(I256{}, N1{});
-        else if(nrhs <= 4)
-            go(I256{}, N4{});
-        else
-            go(I256{}, N8{});
-    }
+        by_nrhs(std::integral_constant<int, 512>{});
     RSA_LAUNCH_CHECK();
     return rocblas_status_success;
 }

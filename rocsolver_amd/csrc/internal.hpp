@@ -29,7 +29,7 @@ rocblas_status getrs_small(hipStream_t stream, bool trans, int n, int nrhs, Batc
 // Square LU for SMALL_MAX < n <= FUSED_MAX, one workgroup per matrix.  AI is
 // read (type TI), the factors are written to AO (type TC; AO may alias AI
 // when TI == TC).  Batch entries with skip[b] != 0 are not touched.
-constexpr int FUSED_MAX = 256;
+constexpr int FUSED_MAX = 512;
 
 template <typename TI, typename TC>
 rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO, int ldo,

@@ -1,8 +1,8 @@
-// lu_fused.hpp -- one-workgroup-per-matrix left-looking LU for 64 < n <= 256.
+// lu_fused.hpp -- one-workgroup-per-matrix left-looking LU for 64 < n <= 512.
 //
 // The whole factorisation of a matrix runs in one launch and one workgroup
-// of NMAX threads (NMAX = 128 or 256), one matrix per workgroup, several
-// workgroups per CU.  Per block column s (width 32):
+// of NMAX threads (NMAX = 128, 256 or 512; one matrix row per thread), one
+// matrix per workgroup, several workgroups per CU.  Per block column s (width 32):
 //   1. gather: the block column is read from the input in its original row
 //      order (coalesced, staged through LDS) and placed in MFMA accumulators
 //      in the current logical row order;
@@ -393,8 +393,11 @@ rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi,
     if(n <= 128)
         hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 128>), dim3(bc), dim3(128), 0, stream, n, AI,
                            ldi, AO, ldo, ipiv, strideP, info, bc, skip);
-    else
+    else if(n <= 256)
         hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 256>), dim3(bc), dim3(256), 0, stream, n, AI,
+                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
+    else
+        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 512>), dim3(bc), dim3(512), 0, stream, n, AI,
                            ldi, AO, ldo, ipiv, strideP, info, bc, skip);
     RSA_LAUNCH_CHECK();
     return rocblas_status_success;

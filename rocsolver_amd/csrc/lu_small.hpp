@@ -27,20 +27,11 @@
 namespace rsa
 {
 
+// arg-max over the team (DPP/permlane reduction, common.hpp)
 template <typename T, int TS>
 __device__ __forceinline__ void team_argmax(T& key, int& lab)
 {
-#pragma unroll
-    for(int off = 1; off < TS; off <<= 1)
-    {
-        T ok = __shfl_xor(key, off, TS);
-        int ol = __shfl_xor(lab, off, TS);
-        if(ok > key || (ok == key && ol < lab))
-        {
-            key = ok;
-            lab = ol;
-        }
-    }
+    group_argmax<TS>(key, lab);
 }
 
 // One team factors one matrix held in registers r[NC].  On return `label`

@@ -57,7 +57,7 @@ def test_getrf_reference_matrices(torch_gpu, handle, m, lda, singular, n, dtype,
 
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 7), (7, 3), (16, 16), (33, 33), (64, 64), (64, 8),
                                  (8, 64), (65, 65), (96, 96), (128, 128), (130, 70), (70, 130),
-                                 (256, 256), (300, 300)])
+                                 (256, 256), (300, 300), (512, 512), (600, 600)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_getrf_random(torch_gpu, handle, m, n, dtype):
     print(f"getrf random m={m} n={n} {dtype.__name__}", flush=True)
@@ -100,7 +100,8 @@ def test_getrf_npvt(torch_gpu, handle, n):
 
 @pytest.mark.parametrize("n,lda,ldb,nrhs", [(20, 20, 20, 10), (30, 50, 30, 20), (30, 30, 50, 30),
                                             (50, 60, 60, 1), (64, 64, 64, 3), (70, 70, 100, 2),
-                                            (130, 130, 130, 9), (192, 192, 200, 1)])
+                                            (130, 130, 130, 9), (192, 192, 200, 1),
+                                            (400, 400, 400, 3), (600, 600, 600, 2)])
 @pytest.mark.parametrize("trans", ["N", "T"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_getrs(torch_gpu, handle, n, lda, ldb, nrhs, trans, dtype):
@@ -125,7 +126,7 @@ def test_getrs(torch_gpu, handle, n, lda, ldb, nrhs, trans, dtype):
 @pytest.mark.parametrize("n,lda,ldb,singular,nrhs", [(20, 20, 20, 0, 10), (30, 50, 30, 1, 20),
                                                      (30, 30, 50, 0, 30), (50, 60, 60, 1, 1),
                                                      (70, 70, 100, 0, 100), (100, 100, 100, 1, 8),
-                                                     (192, 192, 192, 1, 4)])
+                                                     (192, 192, 192, 1, 4), (450, 450, 460, 0, 5)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("batched", [False, True])
 def test_gesv(torch_gpu, handle, n, lda, ldb, singular, nrhs, dtype, batched):

@@ -21,7 +21,7 @@ def dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("n,nrhs,bc", [(96, 16, 6), (128, 4, 5), (256, 8, 6), (200, 1, 4),
+@pytest.mark.parametrize("n,nrhs,bc", [(96, 16, 6), (128, 4, 5), (256, 8, 6), (200, 1, 4), (512, 2, 3),
                                        (70, 3, 3)])
 def test_dsgesv_against_oracle(torch_gpu, handle, n, nrhs, bc):
     print(f"dsgesv n={n} nrhs={nrhs} bc={bc}", flush=True)
