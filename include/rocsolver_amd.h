@@ -237,8 +237,10 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
  *   berr   device or NULL: per matrix max_c ||R_c||_inf / (||A||_inf ||X_c||_inf)
  *          at convergence (-1 where the FP64 path produced X)
  * Workspace comes from the handle (size query supported): about
- * bc * (8 n^2 + 8 n nrhs) bytes.  Supported orders: 64 < n <= 512
- * (rocblas_status_not_implemented otherwise). */
+ * bc * (8 n^2 + 4 n) bytes.  Supported: 64 < n <= 512 and
+ * lda * n * 8 < 2^31 (rocblas_status_not_implemented otherwise).  With
+ * nrhs > 8 the columns are refined in groups of 8: each group iterates until
+ * its own columns pass the test, and iter is the largest group count. */
 rocblas_status rocsolver_amd_dsgesv_strided_batched(
     rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,
     const rocblas_int lda, const rocblas_stride strideA, const double* B, const rocblas_int ldb,
@@ -278,6 +280,59 @@ rocblas_status rocsolver_amd_dgesv_adjoint_strided_batched(
     const rocblas_stride strideDX, double* dB, const rocblas_int lddb,
     const rocblas_stride strideDB, double* dA, const rocblas_int ldda,
     const rocblas_stride strideDA, const rocblas_int batch_count);
+
+/* ------------------------------------------------------- variable size
+ * Batches whose matrices have different orders (0 <= n[b] <= 512).  No
+ * counterpart in the reference.  Sizes are device arrays of batch_count
+ * entries: n, lda (and nrhs, ldb for getrs/gesv); A, B and ipiv are device
+ * arrays of device pointers (ipiv[b] holds n[b] entries).  Per matrix the
+ * result equals the fixed-size routine's (getrf / getrs / gesv semantics,
+ * 1-based ipiv, info = first zero pivot, singular gesv leaves B unchanged).
+ * A matrix with invalid sizes (n < 0, n > 512, lda < max(1,n), nrhs < 0,
+ * ldb < max(1,n), lda*n*sizeof(T) >= 2^31) is skipped with info[b] = -1.
+ * Runs as two launches with no host synchronisation (plan + one persistent
+ * kernel), so it can be captured in a graph once the handle's workspace
+ * (size query supported: about 20 bytes per matrix) is allocated.
+ * getrs takes an optional info array (NULL allowed): 0, or -1 when skipped. */
+rocblas_status rocsolver_amd_sgetrf_vbatched(rocblas_handle handle, const rocblas_int* n,
+                                             float* const A[], const rocblas_int* lda,
+                                             rocblas_int* const ipiv[], rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_amd_dgetrf_vbatched(rocblas_handle handle, const rocblas_int* n,
+                                             double* const A[], const rocblas_int* lda,
+                                             rocblas_int* const ipiv[], rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_amd_sgetrs_vbatched(rocblas_handle handle, const rocblas_operation trans,
+                                             const rocblas_int* n, const rocblas_int* nrhs,
+                                             float* const A[], const rocblas_int* lda,
+                                             rocblas_int* const ipiv[], float* const B[],
+                                             const rocblas_int* ldb, rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_amd_dgetrs_vbatched(rocblas_handle handle, const rocblas_operation trans,
+                                             const rocblas_int* n, const rocblas_int* nrhs,
+                                             double* const A[], const rocblas_int* lda,
+                                             rocblas_int* const ipiv[], double* const B[],
+                                             const rocblas_int* ldb, rocblas_int* info,
+                                             const rocblas_int batch_count);
+rocblas_status rocsolver_amd_sgesv_vbatched(rocblas_handle handle, const rocblas_int* n,
+                                            const rocblas_int* nrhs, float* const A[],
+                                            const rocblas_int* lda, rocblas_int* const ipiv[],
+                                            float* const B[], const rocblas_int* ldb,
+                                            rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_amd_dgesv_vbatched(rocblas_handle handle, const rocblas_int* n,
+                                            const rocblas_int* nrhs, double* const A[],
+                                            const rocblas_int* lda, rocblas_int* const ipiv[],
+                                            double* const B[], const rocblas_int* ldb,
+                                            rocblas_int* info, const rocblas_int batch_count);
+
+/* Schedule table of the variable-size kernel (process-wide), 13 integers:
+ * team[6] (lanes per matrix for orders <= 8, 16, 24, 32, 48, 64: 16, 32 or
+ * 64, at least the order), pack[6] (rounds of teams per work item, 1..64),
+ * workgroups per CU (1..4).  Set by the autotuner (rocsolver_amd/autotune.py);
+ * invalid tables are rejected with rocblas_status_invalid_value. */
+rocblas_status rocsolver_amd_vbatched_set_tuning(const rocblas_int* params,
+                                                 const rocblas_int len);
+rocblas_status rocsolver_amd_vbatched_get_tuning(rocblas_int* params, const rocblas_int len);
 
 /* ---------------------------------------------------------------- version
  * Reference: rocsolver-functions.h:34 and :45; library/src/common/buildinfo.cpp. */

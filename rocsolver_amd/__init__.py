@@ -49,7 +49,24 @@ def lib() -> ctypes.CDLL:
                           "(there is no fallback implementation)")
         _lib = ctypes.CDLL(LIB_PATH)
         _declare(_lib)
+        _install_tuning(_lib)
     return _lib
+
+
+TUNING_PATH = os.path.join(HERE, "tuning", "vbatched_gfx950.json")
+
+
+def _install_tuning(L: ctypes.CDLL) -> None:
+    """Install the autotuned variable-size schedule when one was saved."""
+    if not os.path.exists(TUNING_PATH):
+        return
+    import json
+    with open(TUNING_PATH) as f:
+        vals = [int(v) for v in json.load(f)["table"]]
+    buf = (ctypes.c_int * len(vals))(*vals)
+    st = L.rocsolver_amd_vbatched_set_tuning(ctypes.cast(buf, ctypes.c_void_p), len(vals))
+    if st != 0:
+        raise RocsolverError(f"invalid schedule table in {TUNING_PATH}", st)
 
 
 def rocblas() -> ctypes.CDLL:
@@ -96,6 +113,14 @@ EXTENSIONS = {
                                                     _P, _I, _S, _P, _I, _S, _P, _I, _S, _I],
     "rocsolver_amd_dgesv_adjoint_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S,
                                                     _P, _I, _S, _P, _I, _S, _P, _I, _S, _I],
+    "rocsolver_amd_sgetrf_vbatched": [_P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_dgetrf_vbatched": [_P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_sgetrs_vbatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_dgetrs_vbatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_sgesv_vbatched": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_dgesv_vbatched": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_vbatched_set_tuning": [_P, _I],
+    "rocsolver_amd_vbatched_get_tuning": [_P, _I],
 }
 
 
@@ -273,3 +298,87 @@ def dsgesv(handle: Handle, A, B, want_berr: bool = True):
           n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n, nrhs * n, it.data_ptr(),
           info.data_ptr(), berr.data_ptr() if want_berr else None, bc)
     return X, it, info, berr
+
+
+# ------------------------------------------------- variable-size batches
+VB_TUNING_LEN = 13
+
+
+def vbatched_get_tuning() -> list[int]:
+    buf = (ctypes.c_int * VB_TUNING_LEN)()
+    check("rocsolver_amd_vbatched_get_tuning", ctypes.cast(buf, ctypes.c_void_p), VB_TUNING_LEN)
+    return list(buf)
+
+
+def vbatched_set_tuning(params) -> None:
+    """Install a schedule table (team[6], pack[6], workgroups per CU)."""
+    vals = [int(v) for v in params]
+    buf = (ctypes.c_int * len(vals))(*vals)
+    check("rocsolver_amd_vbatched_set_tuning", ctypes.cast(buf, ctypes.c_void_p), len(vals))
+
+
+class VBatch:
+    """Matrices of different orders packed in one device buffer each for A,
+    B and ipiv, with the device size and pointer arrays the vbatched routines
+    take.  Matrix b is column-major with lda = ldb = n[b]: A_of(b) is a view
+    of shape (n, n) holding A^T (row j = column j), B_of(b) (nrhs, n)."""
+
+    def __init__(self, n, nrhs, dtype, device="cuda"):
+        import numpy as np
+        import torch
+        self.n = np.asarray(n, dtype=np.int64)
+        self.nrhs = np.asarray(nrhs, dtype=np.int64)
+        self.bc = len(self.n)
+        self.dtype, self.device = dtype, device
+        esz = torch.empty((), dtype=dtype).element_size()
+        na = self.n * self.n
+        nb = self.n * self.nrhs
+        self.offA = np.concatenate([[0], np.cumsum(na)[:-1]]).astype(np.int64)
+        self.offB = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
+        self.offP = np.concatenate([[0], np.cumsum(self.n)[:-1]]).astype(np.int64)
+        self.A = torch.zeros(int(na.sum()) + 1, dtype=dtype, device=device)
+        self.B = torch.zeros(int(nb.sum()) + 1, dtype=dtype, device=device)
+        self.ipiv = torch.zeros(int(self.n.sum()) + 1, dtype=torch.int32, device=device)
+        self.info = torch.zeros(self.bc, dtype=torch.int32, device=device)
+        i32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(device)
+        i64 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int64)).to(device)
+        self.n_dev = i32(self.n)
+        self.nrhs_dev = i32(self.nrhs)
+        self.ld_dev = i32(np.maximum(self.n, 1))
+        self.pA = i64(self.A.data_ptr() + self.offA * esz)
+        self.pB = i64(self.B.data_ptr() + self.offB * esz)
+        self.pP = i64(self.ipiv.data_ptr() + self.offP * 4)
+
+    def A_of(self, b: int):
+        k, o = int(self.n[b]), int(self.offA[b])
+        return self.A[o:o + k * k].view(k, k)
+
+    def B_of(self, b: int):
+        k, r, o = int(self.n[b]), int(self.nrhs[b]), int(self.offB[b])
+        return self.B[o:o + k * r].view(r, k)
+
+    def ipiv_of(self, b: int):
+        k, o = int(self.n[b]), int(self.offP[b])
+        return self.ipiv[o:o + k]
+
+    def _p(self) -> str:
+        import torch
+        return "d" if self.dtype == torch.float64 else "s"
+
+    def getrf(self, handle: "Handle") -> None:
+        check(f"rocsolver_amd_{self._p()}getrf_vbatched", handle.value, self.n_dev.data_ptr(),
+              self.pA.data_ptr(), self.ld_dev.data_ptr(), self.pP.data_ptr(),
+              self.info.data_ptr(), self.bc)
+
+    def getrs(self, handle: "Handle", trans: str = "N") -> None:
+        op = {"N": OP_NONE, "T": OP_TRANSPOSE, "C": OP_CONJ_TRANSPOSE}[trans.upper()]
+        check(f"rocsolver_amd_{self._p()}getrs_vbatched", handle.value, op, self.n_dev.data_ptr(),
+              self.nrhs_dev.data_ptr(), self.pA.data_ptr(), self.ld_dev.data_ptr(),
+              self.pP.data_ptr(), self.pB.data_ptr(), self.ld_dev.data_ptr(),
+              self.info.data_ptr(), self.bc)
+
+    def gesv(self, handle: "Handle") -> None:
+        check(f"rocsolver_amd_{self._p()}gesv_vbatched", handle.value, self.n_dev.data_ptr(),
+              self.nrhs_dev.data_ptr(), self.pA.data_ptr(), self.ld_dev.data_ptr(),
+              self.pP.data_ptr(), self.pB.data_ptr(), self.ld_dev.data_ptr(),
+              self.info.data_ptr(), self.bc)

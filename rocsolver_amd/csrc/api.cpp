@@ -90,7 +90,7 @@ rocblas_status getrs_impl(rocblas_handle handle, rocblas_operation trans, int n,
     rocblas_get_stream(handle, &stream);
     if(n <= SMALL_MAX)
         return getrs_small<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true);
-    if(n <= FUSED_MAX)
+    if(n <= FUSED_MAX && fits_buffer<T>(lda, n))
         return getrs_fused<T, T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb, bc,
                                  nullptr, false);
     return getrs_blocked<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
@@ -121,7 +121,7 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
         return reset_info(stream, info, bc);
     if(n <= SMALL_MAX)
         return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
-    if(n <= FUSED_MAX)
+    if(n <= FUSED_MAX && fits_buffer<T>(lda, n))
     {
         rocblas_status st = lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
         if(st != rocblas_status_success)
@@ -184,14 +184,14 @@ rocblas_status dsgesv_impl(rocblas_handle handle, int n, int nrhs, const double*
         rocblas_status st = reset_info(stream, info, bc);
         return st != rocblas_status_success ? st : reset_info(stream, iter, bc);
     }
-    if(n <= SMALL_MAX || n > FUSED_MAX)
+    if(n <= SMALL_MAX || n > FUSED_MAX || !fits_buffer<double>(lda, n))
         return rocblas_status_not_implemented;
     rocblas_device_malloc mem(handle, ws);
     if(!mem)
         return rocblas_status_memory_error;
     return dsgesv_refine(stream, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
                          strided(const_cast<double*>(B), strideB), ldb, strided(X, strideX), ldx,
-                         iter, info, berr, bc, DSGESV_ITERMAX, static_cast<void*>(mem), true);
+                         iter, info, berr, bc, DSGESV_ITERMAX, static_cast<void*>(mem));
 }
 
 __global__ void convert_d2s_kernel(int m, int n, Batch<double> A, int lda, Batch<float> O, int ldo)
@@ -269,6 +269,41 @@ rocblas_status adjoint_impl(rocblas_handle handle, int n, int nrhs, const T* LU,
     return gesv_adjoint<T>(stream, n, nrhs, cb(LU, strideA), lda, ipiv, strideP, cb(X, strideX),
                            ldx, cb(dX, strideDX), lddx, strided(dB, strideDB), lddb,
                            strided(dA, strideDA), ldda, bc);
+}
+
+template <typename T>
+rocblas_status vbatched_impl(rocblas_handle handle, int op, rocblas_operation trans,
+                             const int* n, const int* nrhs, T* const* A, const int* lda,
+                             int* const* ipiv, T* const* B, const int* ldb, int* info, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    bool tr = false;
+    if(op == 1)
+    {
+        rocblas_status st = check_trans(trans, &tr);
+        if(st != rocblas_status_success)
+            return st;
+    }
+    if(bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && bc > 0
+       && (!n || !lda || !A || !ipiv || (op != 0 && (!nrhs || !B || !ldb))
+           || (op != 1 && !info)))
+        return rocblas_status_invalid_pointer;
+    const size_t ws = bc > 0 ? vbatched_workspace_bytes(bc) : 0;
+    if(query)
+        return rocblas_set_optimal_device_memory_size(handle, ws);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    rocblas_device_malloc mem(handle, ws);
+    if(!mem)
+        return rocblas_status_memory_error;
+    return vbatched<T>(stream, op, tr, n, nrhs, A, lda, ipiv, B, ldb, info, bc,
+                       static_cast<void*>(mem));
 }
 
 } // namespace
@@ -448,6 +483,56 @@ rocblas_status rocsolver_amd_dsgetrf_strided_batched(
     }
 RSA_ADJOINT(s, float)
 RSA_ADJOINT(d, double)
+
+#define RSA_VBATCHED(P, T)                                                                      \
+    rocblas_status rocsolver_amd_##P##getrf_vbatched(                                           \
+        rocblas_handle handle, const rocblas_int* n, T* const A[], const rocblas_int* lda,      \
+        rocblas_int* const ipiv[], rocblas_int* info, const rocblas_int batch_count)            \
+    {                                                                                           \
+        return vbatched_impl<T>(handle, 0, rocblas_operation_none, n, nullptr, A, lda, ipiv,    \
+                                nullptr, nullptr, info, batch_count);                           \
+    }                                                                                           \
+    rocblas_status rocsolver_amd_##P##getrs_vbatched(                                           \
+        rocblas_handle handle, const rocblas_operation trans, const rocblas_int* n,             \
+        const rocblas_int* nrhs, T* const A[], const rocblas_int* lda,                          \
+        rocblas_int* const ipiv[], T* const B[], const rocblas_int* ldb, rocblas_int* info,     \
+        const rocblas_int batch_count)                                                          \
+    {                                                                                           \
+        return vbatched_impl<T>(handle, 1, trans, n, nrhs, A, lda, ipiv, B, ldb, info,          \
+                                batch_count);                                                   \
+    }                                                                                           \
+    rocblas_status rocsolver_amd_##P##gesv_vbatched(                                            \
+        rocblas_handle handle, const rocblas_int* n, const rocblas_int* nrhs, T* const A[],     \
+        const rocblas_int* lda, rocblas_int* const ipiv[], T* const B[],                        \
+        const rocblas_int* ldb, rocblas_int* info, const rocblas_int batch_count)               \
+    {                                                                                           \
+        return vbatched_impl<T>(handle, 2, rocblas_operation_none, n, nrhs, A, lda, ipiv, B,    \
+                                ldb, info, batch_count);                                        \
+    }
+RSA_VBATCHED(s, float)
+RSA_VBATCHED(d, double)
+
+rocblas_status rocsolver_amd_vbatched_set_tuning(const rocblas_int* params, const rocblas_int len)
+{
+    return vbatched_set_tuning(params, len) ? rocblas_status_success
+                                            : rocblas_status_invalid_value;
+}
+
+rocblas_status rocsolver_amd_vbatched_get_tuning(rocblas_int* params, const rocblas_int len)
+{
+    if(!params)
+        return rocblas_status_invalid_pointer;
+    if(len != VB_TUNING_LEN)
+        return rocblas_status_invalid_size;
+    const VbTuning t = vbatched_tuning();
+    for(int k = 0; k < VB_BUCKETS; ++k)
+    {
+        params[k] = t.team[k];
+        params[VB_BUCKETS + k] = t.pack[k];
+    }
+    params[2 * VB_BUCKETS] = t.wg_per_cu;
+    return rocblas_status_success;
+}
 
 #define RSA_VERSION_STRING "3.23.0.gfx950"
 

@@ -19,11 +19,62 @@ struct Batch
     T* const* ptrs;
     int64_t stride;
 
+    // Every batch entry is device global memory.  The cast through the
+    // global address space says so to the compiler, so accesses through the
+    // returned pointer are global_* instructions (SGPR base + VGPR offset,
+    // counted by vmcnt only) instead of flat ones, which also hold up every
+    // wait on LDS.
     __device__ __forceinline__ T* operator[](int64_t b) const
     {
-        return ptrs ? ptrs[b] : base + b * stride;
+        T* p = ptrs ? ptrs[b] : base + b * stride;
+        return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
     }
 };
+
+// A column-major matrix (leading dimension ld) seen through a buffer
+// resource.  A load takes a per-lane byte offset (one VGPR) and a uniform
+// byte offset (SGPR), so a thread walking a column block needs no address
+// registers beyond its own offset; a per-lane offset of OUT_OF_RANGE returns
+// 0 without touching memory.  The matrix must span fewer than 2^31 bytes
+// (fits_buffer on the host).
+template <typename T>
+struct MatrixBuffer
+{
+    static constexpr uint32_t OUT_OF_RANGE = 0x80000000u;
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t ld_bytes;
+
+    __device__ __forceinline__ MatrixBuffer(const T* base, int ld, int ncols)
+    {
+        const uint64_t bytes = uint64_t(ld) * uint64_t(ncols) * sizeof(T);
+        // word 3 of a raw gfx9 buffer resource: 32-bit data format, no swizzle
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0,
+                                                 int(bytes < OUT_OF_RANGE ? bytes : OUT_OF_RANGE - 1),
+                                                 0x00020000);
+        ld_bytes = uint32_t(ld) * uint32_t(sizeof(T));
+    }
+    __device__ __forceinline__ uint32_t row_offset(int i) const
+    {
+        return uint32_t(i) * uint32_t(sizeof(T));
+    }
+    __device__ __forceinline__ uint32_t col_offset(int j) const
+    {
+        return uint32_t(j) * ld_bytes;
+    }
+    __device__ __forceinline__ T load(uint32_t voff, uint32_t soff) const
+    {
+        if constexpr(sizeof(T) == 8)
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rsrc, voff, soff, 0));
+        else
+            return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0));
+    }
+};
+
+template <typename T>
+inline bool fits_buffer(int ld, int ncols)
+{
+    return uint64_t(ld) * uint64_t(ncols) * sizeof(T) < (uint64_t(1) << 31);
+}
 
 template <typename T>
 inline Batch<T> strided(T* base, int64_t stride)
@@ -35,6 +86,18 @@ template <typename T>
 inline Batch<T> arrayed(T* const* ptrs)
 {
     return Batch<T>{nullptr, ptrs, 0};
+}
+
+// Orders LDS accesses between the lanes of one wavefront.  A wavefront's
+// LDS instructions execute in issue order, so this only has to stop the
+// compiler from moving memory operations across it (no s_barrier: teams of
+// lanes that never span two waves synchronise with this alone, even when the
+// other waves of the workgroup run different code).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ double fma_t(double a, double b, double c)

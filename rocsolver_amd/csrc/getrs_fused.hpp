@@ -3,12 +3,10 @@
 // library/src/lapack/roclapack_getrs.hpp:117-150 -- laswp, then L, then U for
 // op = N; U^T, L^T, then the reverse interchanges for op = T).
 //
-// Thread i owns row i of NR right-hand sides in registers.  Each triangle is
-// walked in 32-row diagonal blocks: the 32 threads of the block (one half
-// wave) solve it with shuffles (lane k's value reaches the others without
-// LDS or a barrier), publish it to LDS, and every thread below (above)
-// subtracts its L (U) row segment times the block -- one barrier per block.
-// Reads of the LU are coalesced across threads for op = N.
+// Thread i owns row i of NR right-hand sides in registers; the triangles are
+// walked by the row solvers of trsv_rows.hpp (32-row diagonal blocks, loads
+// one block ahead, one barrier per block).  The factors are read through a
+// buffer resource: lda * n * sizeof(TL) must be below 2^31 (fits_buffer).
 //
 // Types: TL is the LU precision and the arithmetic precision; TB is the
 // precision of the right-hand side read from BB and of the solution written
@@ -18,65 +16,28 @@
 #pragma once
 
 #include "internal.hpp"
+#include "trsv_rows.hpp"
 
 #include <type_traits>
 
 namespace rsa
 {
 
-constexpr int GBS = 32; // diagonal block
-
-template <typename TL, typename TB, int NMAX, int NR, bool ACCUM>
-__global__ __launch_bounds__(NMAX) void getrs_fused_kernel(bool trans, int n, int nrhs,
-                                                           Batch<TL> AA, int lda,
-                                                           const int* __restrict__ ipivA,
-                                                           int64_t strideP, Batch<TB> BB, int ldb,
-                                                           Batch<TB> XX, int ldx,
-                                                           const int* __restrict__ skip)
+// Solves one system with the whole workgroup (blockDim.x == NMAX, n <= NMAX).
+template <typename TL, typename TB, int NMAX, int NR, bool ACCUM, bool TRANS>
+__device__ __forceinline__ void getrs_fused_body(int n, int nrhs, const TL* __restrict__ Ap,
+                                                 int lda, const int* __restrict__ ipiv,
+                                                 const TB* Bm, int ldb, TB* X, int ldx)
 {
     __shared__ TL sx[2][GBS][NR];
     __shared__ int sipiv[NMAX];
     __shared__ int ssrc[NMAX];
 
-    const int64_t b = blockIdx.x;
-    if(skip && skip[b] != 0)
-        return;
     const int tid = threadIdx.x;
-    const int lane32 = tid & (GBS - 1);
-    const TL* A = AA[b];
-    const TB* Bm = BB[b];
-    TB* X = XX[b];
-    const int* ipiv = ipivA + b * strideP;
-    auto L = [&](int i, int k) { return A[i + int64_t(k) * lda]; };
-
-    // final position of original row tid after the interchanges (op = N order,
-    // or reverse order for op = T); a pivot outside [0, n) is ignored
-    int p_in = tid < n ? ipiv[tid] - 1 : tid;
-    if(p_in < 0 || p_in >= n)
-        p_in = tid;
-    sipiv[tid] = p_in;
-    __syncthreads();
-    int pos = tid;
-    if(!trans)
-    {
-        for(int k = 0; k < n; ++k)
-        {
-            const int p = sipiv[k];
-            pos = (pos == k) ? p : (pos == p ? k : pos);
-        }
-        ssrc[pos] = tid; // row landing on `pos` comes from original row tid
-    }
-    else
-    {
-        for(int k = n - 1; k >= 0; --k)
-        {
-            const int p = sipiv[k];
-            pos = (pos == k) ? p : (pos == p ? k : pos);
-        }
-    }
-    __syncthreads();
-    const int src = trans ? tid : ssrc[tid];
-    const int nblk = (n + GBS - 1) / GBS;
+    const MatrixBuffer<TL> A(Ap, lda, n);
+    const int pos = pivot_position(n, ipiv, TRANS, sipiv, ssrc);
+    const int src = TRANS ? tid : ssrc[tid];
+    const int dst = TRANS ? pos : tid;
 
     for(int c0 = 0; c0 < nrhs; c0 += NR)
     {
@@ -84,103 +45,11 @@ __global__ __launch_bounds__(NMAX) void getrs_fused_kernel(bool trans, int n, in
 #pragma unroll
         for(int q = 0; q < NR; ++q)
             x[q] = (tid < n && c0 + q < nrhs) ? TL(Bm[src + int64_t(c0 + q) * ldb]) : TL(0);
-        int par = 0;
-
-        // ---- first triangle: L (op = N, unit) or U^T (op = T, non-unit), forward
-        for(int blk = 0; blk < nblk; ++blk, par ^= 1)
-        {
-            const int r0 = blk * GBS;
-            if((tid >> 5) == blk) // the half wave owning rows [r0, r0+32)
-            {
-#pragma unroll
-                for(int k = 0; k < GBS; ++k)
-                {
-                    if(r0 + k < n)
-                    {
-                        if(trans && lane32 == k)
-                        {
-#pragma unroll
-                            for(int q = 0; q < NR; ++q)
-                                x[q] = x[q] / L(r0 + k, r0 + k);
-                        }
-                        const TL lik = (tid < n && lane32 > k)
-                                           ? (trans ? L(r0 + k, tid) : L(tid, r0 + k))
-                                           : TL(0);
-#pragma unroll
-                        for(int q = 0; q < NR; ++q)
-                        {
-                            const TL xk = __shfl(x[q], k, GBS);
-                            if(lane32 > k)
-                                x[q] = fma_t(-lik, xk, x[q]);
-                        }
-                    }
-                }
-#pragma unroll
-                for(int q = 0; q < NR; ++q)
-                    sx[par][lane32][q] = x[q];
-            }
-            __syncthreads();
-            if(tid >= r0 + GBS && tid < n)
-            {
-                const int kend = (n - r0) < GBS ? (n - r0) : GBS;
-                for(int k = 0; k < kend; ++k)
-                {
-                    const TL lik = trans ? L(r0 + k, tid) : L(tid, r0 + k);
-#pragma unroll
-                    for(int q = 0; q < NR; ++q)
-                        x[q] = fma_t(-lik, sx[par][k][q], x[q]);
-                }
-            }
-        }
-
-        // ---- second triangle: U (op = N, non-unit) or L^T (op = T, unit), backward
-        for(int blk = nblk - 1; blk >= 0; --blk, par ^= 1)
-        {
-            const int r0 = blk * GBS;
-            if((tid >> 5) == blk)
-            {
-#pragma unroll
-                for(int k = GBS - 1; k >= 0; --k)
-                {
-                    if(r0 + k < n)
-                    {
-                        if(!trans && lane32 == k)
-                        {
-#pragma unroll
-                            for(int q = 0; q < NR; ++q)
-                                x[q] = x[q] / L(r0 + k, r0 + k);
-                        }
-                        const TL uik = (lane32 < k) ? (trans ? L(r0 + k, tid) : L(tid, r0 + k))
-                                                    : TL(0);
-#pragma unroll
-                        for(int q = 0; q < NR; ++q)
-                        {
-                            const TL xk = __shfl(x[q], k, GBS);
-                            if(lane32 < k)
-                                x[q] = fma_t(-uik, xk, x[q]);
-                        }
-                    }
-                }
-#pragma unroll
-                for(int q = 0; q < NR; ++q)
-                    sx[par][lane32][q] = x[q];
-            }
-            __syncthreads();
-            if(tid < r0)
-            {
-                const int kend = (n - r0) < GBS ? (n - r0) : GBS;
-                for(int k = 0; k < kend; ++k)
-                {
-                    const TL uik = trans ? L(r0 + k, tid) : L(tid, r0 + k);
-#pragma unroll
-                    for(int q = 0; q < NR; ++q)
-                        x[q] = fma_t(-uik, sx[par][k][q], x[q]);
-                }
-            }
-        }
-
-        // ---- store (op = T: apply the interchanges in reverse order)
-        const int dst = trans ? pos : tid;
+        solve_lower_rows<TL, NR, TRANS>(n, A, x, sx);
+        solve_upper_rows<TL, NR, TRANS>(n, A, x, sx);
+        // every thread has read its right-hand side before any row is stored
+        // (X may alias B and op = T stores at the permuted row)
+        __syncthreads();
         if(tid < n)
         {
 #pragma unroll
@@ -191,8 +60,21 @@ __global__ __launch_bounds__(NMAX) void getrs_fused_kernel(bool trans, int n, in
                     o = ACCUM ? TB(o + TB(x[q])) : TB(x[q]);
                 }
         }
-        __syncthreads();
     }
+}
+
+template <typename TL, typename TB, int NMAX, int NR, bool ACCUM, bool TRANS>
+__global__ __launch_bounds__(NMAX) void getrs_fused_kernel(int n, int nrhs, Batch<TL> AA, int lda,
+                                                           const int* __restrict__ ipivA,
+                                                           int64_t strideP, Batch<TB> BB, int ldb,
+                                                           Batch<TB> XX, int ldx,
+                                                           const int* __restrict__ skip)
+{
+    const int64_t b = blockIdx.x;
+    if(skip && skip[b] != 0)
+        return;
+    getrs_fused_body<TL, TB, NMAX, NR, ACCUM, TRANS>(n, nrhs, AA[b], lda, ipivA + b * strideP,
+                                                     BB[b], ldb, XX[b], ldx);
 }
 
 template <typename TL, typename TB, bool ACCUM>
@@ -202,19 +84,26 @@ rocblas_status getrs_fused_launch(hipStream_t stream, bool trans, int n, int nrh
 {
     if(bc <= 0 || n <= 0 || nrhs <= 0)
         return rocblas_status_success;
-    auto go = [&](auto nmax, auto nr) {
+    auto go = [&](auto nmax, auto nr, auto tr) {
         constexpr int NMAX = decltype(nmax)::value;
         constexpr int NR = decltype(nr)::value;
-        hipLaunchKernelGGL((getrs_fused_kernel<TL, TB, NMAX, NR, ACCUM>), dim3(bc), dim3(NMAX), 0,
-                           stream, trans, n, nrhs, A, lda, ipiv, strideP, B, ldb, X, ldx, skip);
+        constexpr bool TR = decltype(tr)::value;
+        hipLaunchKernelGGL((getrs_fused_kernel<TL, TB, NMAX, NR, ACCUM, TR>), dim3(bc), dim3(NMAX),
+                           0, stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, X, ldx, skip);
+    };
+    auto by_trans = [&](auto nmax, auto nr) {
+        if(trans)
+            go(nmax, nr, std::true_type{});
+        else
+            go(nmax, nr, std::false_type{});
     };
     auto by_nrhs = [&](auto nmax) {
         if(nrhs <= 1)
-            go(nmax, std::integral_constant<int, 1>{});
+            by_trans(nmax, std::integral_constant<int, 1>{});
         else if(nrhs <= 4)
-            go(nmax, std::integral_constant<int, 4>{});
+            by_trans(nmax, std::integral_constant<int, 4>{});
         else
-            go(nmax, std::integral_constant<int, 8>{});
+            by_trans(nmax, std::integral_constant<int, 8>{});
     };
     if(n <= 128)
         by_nrhs(std::integral_constant<int, 128>{});

@@ -60,13 +60,11 @@ rocblas_status getrs_blocked(hipStream_t stream, bool trans, int n, int nrhs, Ba
 // ----------------------------------------------- mixed-precision refinement
 // FP32 LU + FP64 refinement with LAPACK dsgesv semantics per matrix, for
 // SMALL_MAX < n <= FUSED_MAX.  `work` must hold dsgesv_workspace_bytes.
-// early_exit: read one device counter per refinement step and stop once
-// every matrix has converged (otherwise the fixed ITERMAX schedule runs).
+// Three launches, no host synchronisation (graph-capturable).
 size_t dsgesv_workspace_bytes(int n, int nrhs, int bc, int itermax);
 rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> A, int lda,
                              Batch<double> B, int ldb, Batch<double> X, int ldx, int* iter,
-                             int* info, double* berr, int bc, int itermax, void* work,
-                             bool early_exit);
+                             int* info, double* berr, int bc, int itermax, void* work);
 
 // ------------------------------------------------------------------ adjoint
 // dB = A^-T dX (transposed solve with the LU and ipiv of A), dA = -dB X^T.
@@ -74,6 +72,34 @@ template <typename T>
 rocblas_status gesv_adjoint(hipStream_t stream, int n, int nrhs, Batch<T> LU, int lda,
                             const int* ipiv, int64_t strideP, Batch<T> X, int ldx, Batch<T> dX,
                             int lddx, Batch<T> dB, int lddb, Batch<T> dA, int ldda, int bc);
+
+// --------------------------------------------------- variable-size batches
+// Orders 0 .. FUSED_MAX mixed in one batch; per-matrix sizes in device
+// arrays.  Six size buckets for n <= 64 (orders 8, 16, 24, 32, 48, 64).
+constexpr int VB_BUCKETS = 6;
+
+// Schedule parameters of the persistent kernel (chosen by the autotuner):
+// team[k]: lanes per matrix in bucket k (16, 32 or 64, at least the bucket's
+// order); pack[k]: rounds of teams per work item (1..64); wg_per_cu:
+// persistent workgroups launched per CU (1..4).
+struct VbTuning
+{
+    int team[VB_BUCKETS];
+    int pack[VB_BUCKETS];
+    int wg_per_cu;
+};
+constexpr int VB_TUNING_LEN = 2 * VB_BUCKETS + 1;
+
+VbTuning vbatched_tuning(); // current process-wide table
+bool vbatched_set_tuning(const int* params, int len);
+size_t vbatched_workspace_bytes(int bc);
+int device_cu_count();
+
+// op: 0 getrf, 1 getrs, 2 gesv.  info may be null for getrs.
+template <typename T>
+rocblas_status vbatched(hipStream_t stream, int op, bool trans, const int* n, const int* nrhs,
+                        T* const* A, const int* lda, int* const* ipiv, T* const* B,
+                        const int* ldb, int* info, int bc, void* work);
 
 // --------------------------------------------------------------- utilities
 rocblas_status reset_info(hipStream_t stream, int* info, int bc);

@@ -36,15 +36,11 @@ namespace rsa
 constexpr int FNB = 32; // block-column width
 constexpr int FHC = 16; // columns per LDS round of the staging/transposition buffer
 
-// At least 2 waves per SIMD: without the bound the compiler spreads into
-// 300-400 registers (1 wave per SIMD); with it the kernel fits 190-240 VGPRs
-// without scratch (a bound of 3 spills), so 2 workgroups of 256 threads
-// share a CU.
+// Factors one matrix with the whole workgroup (blockDim.x == NMAX, n <= NMAX)
+// and returns the first zero pivot (1-based, 0 if none) in every thread.
 template <typename TI, typename TC, int NMAX>
-__global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
-                                                        int ldo, int* __restrict__ ipivA,
-                                                        int64_t strideP, int* __restrict__ infoA,
-                                                        int bc, const int* __restrict__ skip)
+__device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, int ldi,
+                                             TC* __restrict__ Ao, int ldo, int* __restrict__ ipiv)
 {
     using M = mfma16<TC>;
     using acc_t = typename M::acc_t;
@@ -68,15 +64,9 @@ __global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, 
     __shared__ TC skey[2 * NW];
     __shared__ int slab[2 * NW];
 
-    const int64_t b = blockIdx.x;
-    if(skip && skip[b] != 0)
-        return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int w = tid >> 6;
-    const TI* Ai = AI[b];
-    TC* Ao = AO[b];
-    int* ipiv = ipivA + b * strideP;
     const int S = (n + FNB - 1) / FNB;
 
     spi[tid] = tid;
@@ -379,7 +369,25 @@ __global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, 
             __syncthreads();
         }
     }
-    if(tid == 0)
+    return zero_at;
+}
+
+// At least 2 waves per SIMD: without the bound the compiler spreads into
+// 300-400 registers (1 wave per SIMD); with it the kernel fits 190-240 VGPRs
+// without scratch (a bound of 3 spills), so 2 workgroups of 256 threads
+// share a CU.
+template <typename TI, typename TC, int NMAX>
+__global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
+                                                        int ldo, int* __restrict__ ipivA,
+                                                        int64_t strideP, int* __restrict__ infoA,
+                                                        int bc, const int* __restrict__ skip)
+{
+    const int64_t b = blockIdx.x;
+    if(skip && skip[b] != 0)
+        return;
+    const int zero_at
+        = lu_fused_body<TI, TC, NMAX>(n, AI[b], ldi, AO[b], ldo, ipivA + b * strideP);
+    if(threadIdx.x == 0)
         infoA[b] = zero_at;
 }
 

@@ -1,4 +1,4 @@
-// lu_small.hip -- register-resident LU / solve for matrices with m, n <= 64.
+// lu_small.hpp -- register-resident LU / solve for matrices with m, n <= 64.
 //
 // Layout: a "team" of TS lanes (TS = 16, 32 or 64; TS >= m) owns one matrix;
 // a 64-lane wavefront carries 64/TS matrices.  Lane r of a team holds the
@@ -8,8 +8,8 @@
 // Pivoting follows the reference small kernel
 // (library/src/specialized/roclapack_getf2_specialized_kernels.hpp:72-126):
 //   * pivot = first row of max |a| in the column (strict comparison, ties to
-//     the lowest row, NaN never displaces the candidate) -- here a butterfly
-//     arg-max over the team with (key, logical row) pairs;
+//     the lowest row, NaN never displaces the candidate) -- here a DPP /
+//     permlane arg-max over the team with (key, logical row) pairs;
 //   * rows are swapped lazily: lanes never move data, they exchange their
 //     logical row labels, and each lane writes its row to its final label;
 //   * zero pivot -> info = first k+1, no swap, no scaling;
@@ -18,6 +18,12 @@
 // The pivot row is broadcast to the team through LDS (one write by the pivot
 // lane, one broadcast read per element).  Because the operation order equals
 // the unblocked LAPACK-style algorithm, results are bit-identical to it.
+//
+// Every operation is a team-level device function (team_getrf, team_gesv,
+// team_getrs) that synchronises only within its wavefront, so a team's order
+// n may differ from the other teams' (the variable-size batched kernel packs
+// matrices of different orders into one wave); the kernels below run one
+// batch of equal orders, one wavefront per workgroup.
 #pragma once
 
 #include "internal.hpp"
@@ -26,13 +32,6 @@
 
 namespace rsa
 {
-
-// arg-max over the team (DPP/permlane reduction, common.hpp)
-template <typename T, int TS>
-__device__ __forceinline__ void team_argmax(T& key, int& lab)
-{
-    group_argmax<TS>(key, lab);
-}
 
 // One team factors one matrix held in registers r[NC].  On return `label`
 // is this lane's logical row, `mypiv` its ipiv entry (1-based, valid when
@@ -45,14 +44,14 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
 #pragma unroll
     for(int k = 0; k < NC; ++k)
     {
-        if(k < kmax) // uniform over the block
+        if(k < kmax) // uniform over the team
         {
             int plab = k;
             if(PIVOT)
             {
                 T key = (live && label >= k) ? pivot_key(r[k], label == k) : T(-1);
                 int lab = label;
-                team_argmax<T, TS>(key, lab);
+                group_argmax<TS>(key, lab);
                 plab = lab;
             }
             const bool is_piv = (label == plab);
@@ -62,7 +61,7 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
                 for(int j = k; j < NC; ++j)
                     prow[j] = r[j];
             }
-            __syncthreads();
+            wave_sync();
             const T pv = prow[k];
             const bool nz = (pv != T(0));
             if(!nz && zero_at == 0)
@@ -83,33 +82,26 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
                 for(int j = k + 1; j < NC; ++j)
                     r[j] = fma_t(-l, prow[j], r[j]);
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
 
+// getrf of one m x n matrix by the team (row = lane index within the team,
+// valid = the team has a matrix).  prow: LDS, NC elements for this team.
+// Returns the first zero pivot (1-based) or 0, in every lane of the team.
 template <typename T, int NC, int TS, bool PIVOT>
-__global__ __launch_bounds__(64) void getrf_small_kernel(int m, int n, Batch<T> A, int lda,
-                                                         int* __restrict__ ipiv, int64_t strideP,
-                                                         int* __restrict__ info, int bc)
+__device__ __forceinline__ int team_getrf(int row, bool valid, int m, int n, T* Ab, int lda,
+                                          int* ipivb, T* prow)
 {
-    constexpr int TPW = 64 / TS;
-    __shared__ T prow[TPW][NC];
-    const int lane = threadIdx.x;
-    const int team = lane / TS;
-    const int row = lane % TS;
-    const int64_t b = int64_t(blockIdx.x) * TPW + team;
-    const bool valid = b < bc;
     const bool live = valid && row < m;
-    T* Ab = valid ? A[b] : nullptr;
-
     T r[NC];
 #pragma unroll
     for(int j = 0; j < NC; ++j)
         r[j] = (live && j < n) ? Ab[row + int64_t(j) * lda] : T(0);
 
     int label = row, mypiv = row + 1, zero_at = 0;
-    team_lu<T, NC, TS, PIVOT>(r, prow[team], m, n, live, label, mypiv, zero_at);
+    team_lu<T, NC, TS, PIVOT>(r, prow, m, n, live, label, mypiv, zero_at);
 
     const int kmax = m < n ? m : n;
     if(live)
@@ -119,44 +111,32 @@ __global__ __launch_bounds__(64) void getrf_small_kernel(int m, int n, Batch<T> 
             if(j < n)
                 Ab[label + int64_t(j) * lda] = r[j];
         if(PIVOT && label < kmax)
-            ipiv[b * strideP + label] = mypiv;
+            ipivb[label] = mypiv;
     }
-    if(valid && row == 0)
-        info[b] = zero_at;
+    return zero_at;
 }
 
-// Fused gesv: factor in registers, then solve for NR right-hand sides at a
-// time without ever materialising P*B: lane r holds original row r of B, and
-// its label is that row's position in P*B.  Forward (unit L) and backward
-// (non-unit U, true division) substitution run in logical order, the solved
-// value broadcast through LDS -- the column-oriented order of the reference
-// substitution kernels (roclapack_trsm_specialized_kernels.hpp:166-566).
-// Systems with info != 0 leave B untouched (roclapack_gesv.hpp:162-175).
+// Fused gesv of one n x n system by the team: factor in registers, then
+// solve for NR right-hand sides at a time without ever materialising P*B:
+// lane r holds original row r of B, and its label is that row's position in
+// P*B.  Forward (unit L) and backward (non-unit U, true division)
+// substitution run in logical order, the solved value broadcast through LDS
+// -- the column-oriented order of the reference substitution kernels
+// (roclapack_trsm_specialized_kernels.hpp:166-566).  A singular system
+// leaves B untouched (roclapack_gesv.hpp:162-175).  prow: NC, pb: NR LDS
+// elements for this team.  Returns the first zero pivot or 0.
 template <typename T, int NC, int TS, int NR>
-__global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T> A, int lda,
-                                                        int* __restrict__ ipiv, int64_t strideP,
-                                                        Batch<T> B, int ldb,
-                                                        int* __restrict__ info, int bc)
+__device__ __forceinline__ int team_gesv(int row, bool valid, int n, int nrhs, T* Ab, int lda,
+                                         int* ipivb, T* Bb, int ldb, T* prow, T* pb)
 {
-    constexpr int TPW = 64 / TS;
-    __shared__ T prow[TPW][NC];
-    __shared__ T pb[TPW][NR];
-    const int lane = threadIdx.x;
-    const int team = lane / TS;
-    const int row = lane % TS;
-    const int64_t b = int64_t(blockIdx.x) * TPW + team;
-    const bool valid = b < bc;
     const bool live = valid && row < n;
-    T* Ab = valid ? A[b] : nullptr;
-    T* Bb = valid ? B[b] : nullptr;
-
     T r[NC];
 #pragma unroll
     for(int j = 0; j < NC; ++j)
         r[j] = (live && j < n) ? Ab[row + int64_t(j) * lda] : T(0);
 
     int label = row, mypiv = row + 1, zero_at = 0;
-    team_lu<T, NC, TS, true>(r, prow[team], n, n, live, label, mypiv, zero_at);
+    team_lu<T, NC, TS, true>(r, prow, n, n, live, label, mypiv, zero_at);
 
     if(live)
     {
@@ -164,10 +144,8 @@ __global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T
         for(int j = 0; j < NC; ++j)
             if(j < n)
                 Ab[label + int64_t(j) * lda] = r[j];
-        ipiv[b * strideP + label] = mypiv;
+        ipivb[label] = mypiv;
     }
-    if(valid && row == 0)
-        info[b] = zero_at;
 
     const bool solve = live && zero_at == 0;
     for(int c0 = 0; c0 < nrhs; c0 += NR)
@@ -186,16 +164,16 @@ __global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T
                 {
 #pragma unroll
                     for(int q = 0; q < NR; ++q)
-                        pb[team][q] = x[q];
+                        pb[q] = x[q];
                 }
-                __syncthreads();
+                wave_sync();
                 if(label > k)
                 {
 #pragma unroll
                     for(int q = 0; q < NR; ++q)
-                        x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                        x[q] = fma_t(-r[k], pb[q], x[q]);
                 }
-                __syncthreads();
+                wave_sync();
             }
         }
         // U x = y
@@ -210,17 +188,17 @@ __global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T
                     for(int q = 0; q < NR; ++q)
                     {
                         x[q] = x[q] / r[k];
-                        pb[team][q] = x[q];
+                        pb[q] = x[q];
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 if(label < k)
                 {
 #pragma unroll
                     for(int q = 0; q < NR; ++q)
-                        x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                        x[q] = fma_t(-r[k], pb[q], x[q]);
                 }
-                __syncthreads();
+                wave_sync();
             }
         }
         if(solve)
@@ -231,33 +209,22 @@ __global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T
                     Bb[label + int64_t(c0 + q) * ldb] = x[q];
         }
     }
+    return zero_at;
 }
 
-// getrs for n <= 64.  Lane r of the team holds row r of the LU (no
+// getrs of one system by the team.  Lane r holds row r of the LU (no
 // transpose) or column r (transpose), so every substitution step uses only
 // its own registers plus one LDS broadcast.  Row interchanges are composed
 // into one permutation per team (the same swaps, in the same order, as the
 // reference laswp: rocauxiliary_laswp.hpp:41-62) and applied as a gather on
-// load (trans = N) or a scatter on store (trans = T).
+// load (trans = N) or a scatter on store (trans = T).  pb: NR, perm and spiv:
+// TS LDS elements for this team.
 template <typename T, int NC, int TS, int NR>
-__global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int nrhs, Batch<T> A,
-                                                         int lda, const int* __restrict__ ipiv,
-                                                         int64_t strideP, Batch<T> B, int ldb,
-                                                         int bc, bool pivot)
+__device__ __forceinline__ void team_getrs(int row, bool valid, bool trans, int n, int nrhs,
+                                           const T* Ab, int lda, const int* ipivb, T* Bb, int ldb,
+                                           bool pivot, T* pb, int* perm, int* spiv)
 {
-    constexpr int TPW = 64 / TS;
-    __shared__ T pb[TPW][NR];
-    __shared__ int perm[TPW][TS];
-    __shared__ int spiv[TPW][TS];
-    const int lane = threadIdx.x;
-    const int team = lane / TS;
-    const int row = lane % TS;
-    const int64_t b = int64_t(blockIdx.x) * TPW + team;
-    const bool valid = b < bc;
     const bool live = valid && row < n;
-    const T* Ab = valid ? A[b] : nullptr;
-    T* Bb = valid ? B[b] : nullptr;
-
     T r[NC];
     if(!trans)
     {
@@ -274,11 +241,11 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
 
     // position of this lane's row after the interchanges (a pivot outside
     // [0, n) is ignored instead of addressed)
-    int pv_in = (pivot && live) ? ipiv[b * strideP + row] - 1 : row;
+    int pv_in = (pivot && live) ? ipivb[row] - 1 : row;
     if(pv_in < 0 || pv_in >= n)
         pv_in = row;
-    spiv[team][row] = pv_in;
-    __syncthreads();
+    spiv[row] = pv_in;
+    wave_sync();
     int pos = row;
     if(pivot)
     {
@@ -286,7 +253,7 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
         {
             for(int k = 0; k < n; ++k)
             {
-                const int p = spiv[team][k];
+                const int p = spiv[k];
                 if(pos == k)
                     pos = p;
                 else if(pos == p)
@@ -297,7 +264,7 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
         {
             for(int k = n - 1; k >= 0; --k)
             {
-                const int p = spiv[team][k];
+                const int p = spiv[k];
                 if(pos == k)
                     pos = p;
                 else if(pos == p)
@@ -305,9 +272,9 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
             }
         }
     }
-    perm[team][pos] = row; // row that lands on position pos
-    __syncthreads();
-    const int src = (!trans && live) ? perm[team][row] : row;
+    perm[pos] = row; // row that lands on position pos
+    wave_sync();
+    const int src = (!trans && live) ? perm[row] : row;
 
     for(int c0 = 0; c0 < nrhs; c0 += NR)
     {
@@ -326,16 +293,16 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            pb[team][q] = x[q];
+                            pb[q] = x[q];
                     }
-                    __syncthreads();
+                    wave_sync();
                     if(row > k)
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                            x[q] = fma_t(-r[k], pb[q], x[q]);
                     }
-                    __syncthreads();
+                    wave_sync();
                 }
             }
 #pragma unroll
@@ -349,19 +316,21 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
                         for(int q = 0; q < NR; ++q)
                         {
                             x[q] = x[q] / r[k];
-                            pb[team][q] = x[q];
+                            pb[q] = x[q];
                         }
                     }
-                    __syncthreads();
+                    wave_sync();
                     if(row < k)
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                            x[q] = fma_t(-r[k], pb[q], x[q]);
                     }
-                    __syncthreads();
+                    wave_sync();
                 }
             }
+            // every lane of the team has read its right-hand side (the gather
+            // above) before any row is stored
             if(live)
             {
 #pragma unroll
@@ -384,17 +353,17 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
                         for(int q = 0; q < NR; ++q)
                         {
                             x[q] = x[q] / r[k];
-                            pb[team][q] = x[q];
+                            pb[q] = x[q];
                         }
                     }
-                    __syncthreads();
+                    wave_sync();
                     if(row > k)
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                            x[q] = fma_t(-r[k], pb[q], x[q]);
                     }
-                    __syncthreads();
+                    wave_sync();
                 }
             }
             // L^T x = y: r[k] = L[k][r] for k > r
@@ -407,16 +376,16 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            pb[team][q] = x[q];
+                            pb[q] = x[q];
                     }
-                    __syncthreads();
+                    wave_sync();
                     if(row < k)
                     {
 #pragma unroll
                         for(int q = 0; q < NR; ++q)
-                            x[q] = fma_t(-r[k], pb[team][q], x[q]);
+                            x[q] = fma_t(-r[k], pb[q], x[q]);
                     }
-                    __syncthreads();
+                    wave_sync();
                 }
             }
             if(live)
@@ -428,6 +397,69 @@ __global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int 
             }
         }
     }
+}
+
+// ------------------------------------------------------------- kernels
+// One wavefront per workgroup, 64/TS matrices of the same order per wave.
+
+template <typename T, int NC, int TS, bool PIVOT>
+__global__ __launch_bounds__(64) void getrf_small_kernel(int m, int n, Batch<T> A, int lda,
+                                                         int* __restrict__ ipiv, int64_t strideP,
+                                                         int* __restrict__ info, int bc)
+{
+    constexpr int TPW = 64 / TS;
+    __shared__ T prow[TPW][NC];
+    const int lane = threadIdx.x;
+    const int team = lane / TS;
+    const int row = lane % TS;
+    const int64_t b = int64_t(blockIdx.x) * TPW + team;
+    const bool valid = b < bc;
+    const int zero_at = team_getrf<T, NC, TS, PIVOT>(row, valid, m, n, valid ? A[b] : nullptr,
+                                                     lda, ipiv + (valid ? b : 0) * strideP,
+                                                     prow[team]);
+    if(valid && row == 0)
+        info[b] = zero_at;
+}
+
+template <typename T, int NC, int TS, int NR>
+__global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T> A, int lda,
+                                                        int* __restrict__ ipiv, int64_t strideP,
+                                                        Batch<T> B, int ldb,
+                                                        int* __restrict__ info, int bc)
+{
+    constexpr int TPW = 64 / TS;
+    __shared__ T prow[TPW][NC];
+    __shared__ T pb[TPW][NR];
+    const int lane = threadIdx.x;
+    const int team = lane / TS;
+    const int row = lane % TS;
+    const int64_t b = int64_t(blockIdx.x) * TPW + team;
+    const bool valid = b < bc;
+    const int zero_at = team_gesv<T, NC, TS, NR>(
+        row, valid, n, nrhs, valid ? A[b] : nullptr, lda, ipiv + (valid ? b : 0) * strideP,
+        valid ? B[b] : nullptr, ldb, prow[team], pb[team]);
+    if(valid && row == 0)
+        info[b] = zero_at;
+}
+
+template <typename T, int NC, int TS, int NR>
+__global__ __launch_bounds__(64) void getrs_small_kernel(bool trans, int n, int nrhs, Batch<T> A,
+                                                         int lda, const int* __restrict__ ipiv,
+                                                         int64_t strideP, Batch<T> B, int ldb,
+                                                         int bc, bool pivot)
+{
+    constexpr int TPW = 64 / TS;
+    __shared__ T pb[TPW][NR];
+    __shared__ int perm[TPW][TS];
+    __shared__ int spiv[TPW][TS];
+    const int lane = threadIdx.x;
+    const int team = lane / TS;
+    const int row = lane % TS;
+    const int64_t b = int64_t(blockIdx.x) * TPW + team;
+    const bool valid = b < bc;
+    team_getrs<T, NC, TS, NR>(row, valid, trans, n, nrhs, valid ? A[b] : nullptr, lda,
+                              ipiv + (valid ? b : 0) * strideP, valid ? B[b] : nullptr, ldb, pivot,
+                              pb[team], perm[team], spiv[team]);
 }
 
 // ------------------------------------------------------------- dispatch

@@ -92,3 +92,42 @@ def test_gesv_and_laswp_invalid_size(p):
     for n, lda, k1, k2, incx in [(-1, 4, 1, 2, 1), (4, 0, 1, 2, 1), (4, 4, 1, 2, 0),
                                  (4, 4, 0, 2, 1), (4, 4, 1, 0, 1), (4, 4, 3, 2, 1)]:
         assert ra.call(f"rocsolver_{p}laswp", DUMMY, n, DUMMY, lda, k1, k2, DUMMY, incx) == 4
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_vbatched_argument_checks(p):
+    # null handle first, then trans (getrs), then batch_count < 0
+    assert ra.call(f"rocsolver_amd_{p}getrf_vbatched", None, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
+                   1) == 1
+    assert ra.call(f"rocsolver_amd_{p}gesv_vbatched", None, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
+                   DUMMY, DUMMY, DUMMY, 1) == 1
+    assert ra.call(f"rocsolver_amd_{p}getrs_vbatched", DUMMY, 999, DUMMY, DUMMY, DUMMY, DUMMY,
+                   DUMMY, DUMMY, DUMMY, None, -1) == 11
+    assert ra.call(f"rocsolver_amd_{p}getrf_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
+                   -1) == 4
+    assert ra.call(f"rocsolver_amd_{p}gesv_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
+                   DUMMY, DUMMY, DUMMY, -1) == 4
+
+
+def test_vbatched_tuning_table():
+    default = ra.vbatched_get_tuning()
+    assert len(default) == ra.VB_TUNING_LEN
+    try:
+        new = [64, 32, 64, 64, 64, 64, 1, 3, 8, 64, 2, 1, 2]
+        ra.vbatched_set_tuning(new)
+        assert ra.vbatched_get_tuning() == new
+        bad = [
+            [8, 16, 32, 32, 64, 64, 2, 2, 2, 2, 2, 2, 1],   # team 8 is not a team size
+            [16, 16, 16, 32, 64, 64, 2, 2, 2, 2, 2, 2, 1],  # 16 lanes cannot hold order 24
+            [16, 16, 32, 32, 64, 64, 0, 2, 2, 2, 2, 2, 1],  # pack 0
+            [16, 16, 32, 32, 64, 64, 2, 2, 2, 2, 2, 65, 1],  # pack 65
+            [16, 16, 32, 32, 64, 64, 2, 2, 2, 2, 2, 2, 5],  # 5 workgroups per CU
+        ]
+        for t in bad:
+            with pytest.raises(ra.RocsolverError):
+                ra.vbatched_set_tuning(t)
+            assert ra.vbatched_get_tuning() == new
+        with pytest.raises(ra.RocsolverError):
+            ra.vbatched_set_tuning(new[:-1])
+    finally:
+        ra.vbatched_set_tuning(default)

@@ -1,0 +1,183 @@
+"""GPU checks of the variable-size batched routines
+(rocsolver_amd_{s,d}{getrf,getrs,gesv}_vbatched).
+
+Every matrix runs through the kernels of the fixed-size path for its order,
+so each result must equal, bit for bit, what the fixed-size strided routine
+gives that matrix alone; the fixed-size routines are themselves pinned
+against the oracle (test_gpu_lu.py).  The oracle checks here cover the
+pivots (exact) and the solution (tolerance) directly as well.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import rocsolver_amd as ra
+
+pytestmark = pytest.mark.gpu
+
+# every bucket edge and the large-path boundaries
+ORDERS = [1, 2, 5, 8, 9, 13, 16, 17, 20, 24, 25, 31, 32, 33, 40, 48, 49, 55, 64, 65, 70, 100,
+          127, 128, 129, 200, 255, 256, 257, 300, 384, 511, 512]
+
+
+def make(seed, orders, dtype):
+    g = np.random.default_rng(seed)
+    n = np.array(orders, dtype=np.int64)
+    nrhs = g.integers(1, 7, size=len(n))  # up to 6: more than one 4-column pass
+    As = [g.uniform(-1, 1, (k, k)) + k * np.eye(k) * (g.uniform() < 0.7) for k in n]
+    Bs = [g.uniform(-1, 1, (r, k)) for k, r in zip(n, nrhs)]
+    return n, nrhs, [a.astype(dtype) for a in As], [b.astype(dtype) for b in Bs]
+
+
+def load(torch, n, nrhs, As, Bs, dtype):
+    vb = ra.VBatch(n, nrhs, dtype=getattr(torch, dtype))
+    for b in range(len(n)):
+        vb.A_of(b).copy_(torch.from_numpy(As[b]))
+        vb.B_of(b).copy_(torch.from_numpy(Bs[b]))
+    return vb
+
+
+def fixed_gesv(torch, handle, a, bm):
+    A = torch.from_numpy(a[None].copy()).cuda()
+    B = torch.from_numpy(bm[None].copy()).cuda()
+    ipiv, info = ra.gesv(handle, A, B)
+    torch.cuda.synchronize()
+    return A[0].cpu().numpy(), B[0].cpu().numpy(), ipiv[0].cpu().numpy(), int(info[0])
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_gesv_vbatched_equals_fixed_size(torch_gpu, handle, dtype):
+    torch = torch_gpu
+    order = list(np.random.default_rng(3).permutation(ORDERS * 2))
+    n, nrhs, As, Bs = make(11, order, dtype)
+    print(f"gesv_vbatched {dtype} bc={len(n)}", flush=True)
+    vb = load(torch, n, nrhs, As, Bs, dtype)
+    vb.gesv(handle)
+    torch.cuda.synchronize()
+    info = vb.info.cpu().numpy()
+    for b in range(len(n)):
+        LU, X, ipiv, inf = fixed_gesv(torch, handle, As[b], Bs[b])
+        np.testing.assert_array_equal(vb.A_of(b).cpu().numpy(), LU, err_msg=f"n={n[b]}")
+        np.testing.assert_array_equal(vb.B_of(b).cpu().numpy(), X, err_msg=f"n={n[b]}")
+        np.testing.assert_array_equal(vb.ipiv_of(b).cpu().numpy(), ipiv)
+        assert info[b] == inf
+        # and against the oracle directly
+        k = int(n[b])
+        Ao = As[b][None].copy()
+        ipo, info_o, gap = oracle.getf2(Ao, k, with_gap=True)
+        if (gap > 1e3 * k * np.finfo(As[b].dtype).eps).all():  # unambiguous pivots
+            np.testing.assert_array_equal(ipiv, ipo[0])
+        assert info[b] == info_o[0]
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_getrf_getrs_vbatched_equal_fixed_size(torch_gpu, handle, dtype, trans):
+    torch = torch_gpu
+    n, nrhs, As, Bs = make(12, ORDERS, dtype)
+    print(f"getrf/getrs_vbatched {dtype} {trans} bc={len(n)}", flush=True)
+    vb = load(torch, n, nrhs, As, Bs, dtype)
+    vb.getrf(handle)
+    vb.getrs(handle, trans)
+    torch.cuda.synchronize()
+    info = vb.info.cpu().numpy()
+    assert np.all(info == 0)  # getrs writes 0 for every processed matrix
+    for b in range(len(n)):
+        A = torch.from_numpy(As[b][None].copy()).cuda()
+        B = torch.from_numpy(Bs[b][None].copy()).cuda()
+        ipiv, _ = ra.getrf(handle, A)
+        ra.getrs(handle, A, ipiv, B, trans)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(vb.A_of(b).cpu().numpy(), A[0].cpu().numpy())
+        np.testing.assert_array_equal(vb.ipiv_of(b).cpu().numpy(), ipiv[0].cpu().numpy())
+        np.testing.assert_array_equal(vb.B_of(b).cpu().numpy(), B[0].cpu().numpy(),
+                                      err_msg=f"n={n[b]}")
+
+
+def test_gesv_vbatched_solution_accuracy(torch_gpu, handle):
+    torch = torch_gpu
+    n, nrhs, As, Bs = make(13, ORDERS, "float64")
+    vb = load(torch, n, nrhs, As, Bs, "float64")
+    vb.gesv(handle)
+    torch.cuda.synchronize()
+    for b in range(len(n)):
+        k = int(n[b])
+        X = vb.B_of(b).cpu().numpy()
+        Xo, _, info_o = oracle.gesv(As[b][None], k, Bs[b][None])
+        if info_o[0] == 0:
+            err = np.abs(X - Xo[0]).max() / max(np.abs(Xo[0]).max(), 1e-300)
+            assert err <= 1e3 * k * np.finfo(np.float64).eps, (k, err)
+
+
+def test_vbatched_invalid_and_empty_entries(torch_gpu, handle):
+    torch = torch_gpu
+    n = np.array([16, 0, 100, 40, 300, 8])
+    nrhs = np.array([2, 1, 3, 0, 1, 1])
+    vb = ra.VBatch(n, nrhs, dtype=torch.float64)
+    g = np.random.default_rng(5)
+    for b in range(len(n)):
+        k = int(n[b])
+        vb.A_of(b).copy_(torch.from_numpy(g.uniform(-1, 1, (k, k)) + k * np.eye(k)))
+        vb.B_of(b).copy_(torch.from_numpy(g.uniform(-1, 1, (int(nrhs[b]), k))))
+    A0, B0 = vb.A.clone(), vb.B.clone()
+    # invalid sizes: order 600 (> 512) and lda < n, set through the device arrays
+    vb.n_dev[2] = 600
+    vb.ld_dev[4] = 10
+    vb.gesv(handle)
+    torch.cuda.synchronize()
+    info = vb.info.cpu().numpy()
+    assert info[2] == -1 and info[4] == -1
+    assert info[1] == 0 and info[3] == 0  # n = 0 and nrhs = 0: nothing to do
+    # skipped matrices are untouched
+    for b in (2, 3, 4):
+        o, k = int(vb.offA[b]), int(n[b])
+        np.testing.assert_array_equal(vb.A[o:o + k * k].cpu().numpy(),
+                                      A0[o:o + k * k].cpu().numpy())
+    # the others are solved
+    for b in (0, 5):
+        assert info[b] == 0
+        k, r = int(n[b]), int(nrhs[b])
+        a = A0[int(vb.offA[b]):int(vb.offA[b]) + k * k].view(k, k).cpu().numpy()
+        rhs = B0[int(vb.offB[b]):int(vb.offB[b]) + k * r].view(r, k).cpu().numpy()
+        X = vb.B_of(b).cpu().numpy()
+        assert oracle.solve_backward_error(a[None], k, X[None], rhs[None]).max() \
+            <= 10 * k * np.finfo(np.float64).eps
+
+
+def test_vbatched_singular_leaves_b(torch_gpu, handle):
+    torch = torch_gpu
+    n = np.array([20, 150, 20, 150])
+    nrhs = np.array([2, 2, 2, 2])
+    g = np.random.default_rng(8)
+    As = [g.uniform(-1, 1, (k, k)) + k * np.eye(k) for k in n]
+    As[1][7, :] = 0.0  # zero column 8 (row 7 of A^T)
+    As[2][3, :] = 0.0
+    Bs = [g.uniform(-1, 1, (r, k)) for k, r in zip(n, nrhs)]
+    vb = load(torch, n, nrhs, As, Bs, "float64")
+    vb.gesv(handle)
+    torch.cuda.synchronize()
+    info = vb.info.cpu().numpy()
+    assert info[1] == 8 and info[2] == 4
+    np.testing.assert_array_equal(vb.B_of(1).cpu().numpy(), Bs[1])
+    np.testing.assert_array_equal(vb.B_of(2).cpu().numpy(), Bs[2])
+    assert info[0] == 0 and info[3] == 0
+
+
+def test_vbatched_tuning_does_not_change_results(torch_gpu, handle):
+    torch = torch_gpu
+    n, nrhs, As, Bs = make(14, ORDERS, "float64")
+    default = ra.vbatched_get_tuning()
+    out = []
+    try:
+        for table in (default, [64, 64, 64, 64, 64, 64, 1, 1, 1, 1, 1, 1, 2],
+                      [32, 16, 32, 64, 64, 64, 8, 3, 5, 1, 2, 7, 1]):
+            ra.vbatched_set_tuning(table)
+            vb = load(torch, n, nrhs, As, Bs, "float64")
+            vb.gesv(handle)
+            torch.cuda.synchronize()
+            out.append((vb.A.cpu().numpy(), vb.B.cpu().numpy(), vb.ipiv.cpu().numpy()))
+    finally:
+        ra.vbatched_set_tuning(default)
+    for o in out[1:]:
+        for x, y in zip(out[0], o):
+            np.testing.assert_array_equal(x, y)
