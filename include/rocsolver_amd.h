@@ -239,13 +239,36 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
  * Workspace comes from the handle (size query supported): about
  * bc * (8 n^2 + 4 n) bytes.  Supported: 64 < n <= 512 and
  * lda * n * 8 < 2^31 (rocblas_status_not_implemented otherwise).  With
- * nrhs > 8 the columns are refined in groups of 8: each group iterates until
- * its own columns pass the test, and iter is the largest group count. */
+ * nrhs > 8 (> 4 when n > 256) the columns are refined in groups of 8 (4):
+ * each group iterates until its own columns pass the test, and iter is the
+ * largest group count. */
 rocblas_status rocsolver_amd_dsgesv_strided_batched(
     rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,
     const rocblas_int lda, const rocblas_stride strideA, const double* B, const rocblas_int ldb,
     const rocblas_stride strideB, double* X, const rocblas_int ldx, const rocblas_stride strideX,
     rocblas_int* iter, rocblas_int* info, double* berr, const rocblas_int batch_count);
+
+/* Mixed-precision solve with a choice of factorisation (no counterpart in
+ * the reference; same arguments, outputs and fallback as
+ * rocsolver_amd_dsgesv_strided_batched):
+ *   factor 0: FP32 LU of A (identical to rocsolver_amd_dsgesv_strided_batched);
+ *   factor 1: A is equilibrated with power-of-two row and column scales R, C,
+ *             and R A C is factored in FP32 with the trailing updates on the
+ *             matrix cores in FP16 operands (FP32 accumulation);
+ *   factor 2: the same with BF16 operands.
+ * For factors 1 and 2 each correction solves (R A C) y = R r and adds C y; the
+ * residual and the stopping test use A itself in FP64, and entries of A beyond
+ * the FP32 range are not a failure (iter = -2 only for an out-of-range scaled
+ * right-hand side).  Refinement converges when the half-width factors are
+ * accurate enough for the matrix (roughly kappa(A) * 2^-11 < 1 for FP16,
+ * kappa(A) * 2^-8 < 1 for BF16); other systems take the FP64 path (iter =
+ * -31).  An invalid factor gives rocblas_status_invalid_value. */
+rocblas_status rocsolver_amd_dxgesv_strided_batched(
+    rocblas_handle handle, const rocblas_int factor, const rocblas_int n, const rocblas_int nrhs,
+    const double* A, const rocblas_int lda, const rocblas_stride strideA, const double* B,
+    const rocblas_int ldb, const rocblas_stride strideB, double* X, const rocblas_int ldx,
+    const rocblas_stride strideX, rocblas_int* iter, rocblas_int* info, double* berr,
+    const rocblas_int batch_count);
 
 /* FP32 LU of FP64 matrices (the factorisation step of the mixed-precision
  * solve, usable on its own).  No counterpart in the reference.  A (FP64,

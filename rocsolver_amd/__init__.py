@@ -119,6 +119,8 @@ EXTENSIONS = {
     "rocsolver_amd_dgetrs_vbatched": [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _I],
     "rocsolver_amd_sgesv_vbatched": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I],
     "rocsolver_amd_dgesv_vbatched": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I],
+    "rocsolver_amd_dxgesv_strided_batched": [_P, _I, _I, _I, _P, _I, _S, _P, _I, _S, _P, _I, _S,
+                                             _P, _P, _P, _I],
     "rocsolver_amd_vbatched_set_tuning": [_P, _I],
     "rocsolver_amd_vbatched_get_tuning": [_P, _I],
 }
@@ -281,8 +283,13 @@ def gesv(handle: Handle, A, B, batched: bool = False):
     return ipiv[:, :n], info
 
 
-def dsgesv(handle: Handle, A, B, want_berr: bool = True):
-    """Mixed-precision solve of every system (FP32 LU + FP64 refinement).
+FACTOR = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+
+def dsgesv(handle: Handle, A, B, want_berr: bool = True, factor: str = "fp32"):
+    """Mixed-precision solve of every system (FP32 LU + FP64 refinement; with
+    factor "fp16"/"bf16" the equilibrated LU uses half-width matrix-core
+    operands).
 
     A (bc, n, lda) and B (bc, nrhs, ldb) are float64 and left unchanged.
     Returns (X (bc, nrhs, n), iter[bc], info[bc], berr[bc] or None)."""
@@ -294,9 +301,15 @@ def dsgesv(handle: Handle, A, B, want_berr: bool = True):
     it = torch.zeros(bc, dtype=torch.int32, device=dev)
     info = torch.zeros(bc, dtype=torch.int32, device=dev)
     berr = torch.zeros(bc, dtype=torch.float64, device=dev) if want_berr else None
-    check("rocsolver_amd_dsgesv_strided_batched", handle.value, n, nrhs, A.data_ptr(), lda,
-          n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n, nrhs * n, it.data_ptr(),
-          info.data_ptr(), berr.data_ptr() if want_berr else None, bc)
+    if factor == "fp32":
+        check("rocsolver_amd_dsgesv_strided_batched", handle.value, n, nrhs, A.data_ptr(), lda,
+              n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n, nrhs * n, it.data_ptr(),
+              info.data_ptr(), berr.data_ptr() if want_berr else None, bc)
+    else:
+        check("rocsolver_amd_dxgesv_strided_batched", handle.value, FACTOR[factor], n, nrhs,
+              A.data_ptr(), lda, n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n,
+              nrhs * n, it.data_ptr(), info.data_ptr(),
+              berr.data_ptr() if want_berr else None, bc)
     return X, it, info, berr
 
 

@@ -159,12 +159,15 @@ rocblas_status laswp_impl(rocblas_handle handle, int n, T* A, int lda, int k1, i
 
 constexpr int DSGESV_ITERMAX = 30; // LAPACK dsgesv ITERMAX
 
-rocblas_status dsgesv_impl(rocblas_handle handle, int n, int nrhs, const double* A, int lda,
-                           int64_t strideA, const double* B, int ldb, int64_t strideB, double* X,
-                           int ldx, int64_t strideX, int* iter, int* info, double* berr, int bc)
+rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, const double* A,
+                           int lda, int64_t strideA, const double* B, int ldb, int64_t strideB,
+                           double* X, int ldx, int64_t strideX, int* iter, int* info,
+                           double* berr, int bc)
 {
     if(!handle)
         return rocblas_status_invalid_handle;
+    if(factor < 0 || factor > 2)
+        return rocblas_status_invalid_value;
     if(n < 0 || nrhs < 0 || lda < n || ldb < n || ldx < n || bc < 0)
         return rocblas_status_invalid_size;
     const bool query = rocblas_is_device_memory_size_query(handle);
@@ -189,7 +192,7 @@ rocblas_status dsgesv_impl(rocblas_handle handle, int n, int nrhs, const double*
     rocblas_device_malloc mem(handle, ws);
     if(!mem)
         return rocblas_status_memory_error;
-    return dsgesv_refine(stream, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
+    return dsgesv_refine(stream, factor, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
                          strided(const_cast<double*>(B), strideB), ldb, strided(X, strideX), ldx,
                          iter, info, berr, bc, DSGESV_ITERMAX, static_cast<void*>(mem));
 }
@@ -453,8 +456,19 @@ rocblas_status rocsolver_amd_dsgesv_strided_batched(
     const rocblas_stride strideB, double* X, const rocblas_int ldx, const rocblas_stride strideX,
     rocblas_int* iter, rocblas_int* info, double* berr, const rocblas_int batch_count)
 {
-    return dsgesv_impl(handle, n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, iter,
+    return dsgesv_impl(handle, 0, n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX, iter,
                        info, berr, batch_count);
+}
+
+rocblas_status rocsolver_amd_dxgesv_strided_batched(
+    rocblas_handle handle, const rocblas_int factor, const rocblas_int n, const rocblas_int nrhs,
+    const double* A, const rocblas_int lda, const rocblas_stride strideA, const double* B,
+    const rocblas_int ldb, const rocblas_stride strideB, double* X, const rocblas_int ldx,
+    const rocblas_stride strideX, rocblas_int* iter, rocblas_int* info, double* berr,
+    const rocblas_int batch_count)
+{
+    return dsgesv_impl(handle, factor, n, nrhs, A, lda, strideA, B, ldb, strideB, X, ldx, strideX,
+                       iter, info, berr, batch_count);
 }
 
 rocblas_status rocsolver_amd_dsgetrf_strided_batched(

@@ -88,6 +88,19 @@ inline Batch<T> arrayed(T* const* ptrs)
     return Batch<T>{nullptr, ptrs, 0};
 }
 
+// Value of v in lane l (l uniform across the wavefront) through v_readlane:
+// a few cycles, no LDS.
+__device__ __forceinline__ float readlane_t(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_t(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
 // Orders LDS accesses between the lanes of one wavefront.  A wavefront's
 // LDS instructions execute in issue order, so this only has to stop the
 // compiler from moving memory operations across it (no s_barrier: teams of

@@ -36,6 +36,13 @@ rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<
                         int* ipiv, int64_t strideP, int* info, int bc,
                         const int* skip = nullptr);
 
+// FP32 LU of diag(rs) A diag(cs) (A FP64; the scales of matrix b at
+// rs + b * strideS, cs + b * strideS) whose trailing updates run on the matrix
+// cores with FP16 (opk = 1) or BF16 (opk = 2) operands and FP32 accumulation.
+rocblas_status lu_fused_lowp(hipStream_t stream, int opk, int n, Batch<double> AI, int ldi,
+                             Batch<float> AO, int ldo, int* ipiv, int64_t strideP, int* info,
+                             int bc, const double* rs, const double* cs, int64_t strideS);
+
 // One-workgroup-per-system solve with the LU of getrf for SMALL_MAX < n <=
 // FUSED_MAX, arithmetic in TL; reads the right-hand sides from B (type TB),
 // writes (or with accum, adds) the solution to X (may alias B).  Entries with
@@ -61,10 +68,12 @@ rocblas_status getrs_blocked(hipStream_t stream, bool trans, int n, int nrhs, Ba
 // FP32 LU + FP64 refinement with LAPACK dsgesv semantics per matrix, for
 // SMALL_MAX < n <= FUSED_MAX.  `work` must hold dsgesv_workspace_bytes.
 // Three launches, no host synchronisation (graph-capturable).
+// factor: 0 = FP32 LU of A (LAPACK dsgesv); 1 / 2 = FP32 LU of the
+// equilibrated matrix with FP16 / BF16 matrix-core operands in the updates.
 size_t dsgesv_workspace_bytes(int n, int nrhs, int bc, int itermax);
-rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> A, int lda,
-                             Batch<double> B, int ldb, Batch<double> X, int ldx, int* iter,
-                             int* info, double* berr, int bc, int itermax, void* work);
+rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Batch<double> A,
+                             int lda, Batch<double> B, int ldb, Batch<double> X, int ldx,
+                             int* iter, int* info, double* berr, int bc, int itermax, void* work);
 
 // ------------------------------------------------------------------ adjoint
 // dB = A^-T dX (transposed solve with the LU and ipiv of A), dA = -dB X^T.

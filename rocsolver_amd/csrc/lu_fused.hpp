@@ -30,6 +30,8 @@
 #include "internal.hpp"
 #include "mfma.hpp"
 
+#include <type_traits>
+
 namespace rsa
 {
 
@@ -38,10 +40,19 @@ constexpr int FHC = 16; // columns per LDS round of the staging/transposition bu
 
 // Factors one matrix with the whole workgroup (blockDim.x == NMAX, n <= NMAX)
 // and returns the first zero pivot (1-based, 0 if none) in every thread.
-template <typename TI, typename TC, int NMAX>
+// With row and column scales (rs, cs non-null) the matrix factored is
+// diag(rs) A diag(cs), formed in TI arithmetic as each entry is read.  OPK
+// selects the matrix-core operands of the trailing updates: the factor
+// precision itself, or FP16 / BF16 operands with FP32 accumulation (TC =
+// float; the panel, the block solves and the stored factors stay FP32).
+template <typename TI, typename TC, int NMAX, int OPK = MFMA_NATIVE>
 __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, int ldi,
-                                             TC* __restrict__ Ao, int ldo, int* __restrict__ ipiv)
+                                             TC* __restrict__ Ao, int ldo, int* __restrict__ ipiv,
+                                             const double* __restrict__ rs = nullptr,
+                                             const double* __restrict__ cs = nullptr)
 {
+    static_assert(OPK == MFMA_NATIVE || std::is_same<TC, float>::value,
+                  "half-width operands accumulate in FP32");
     using M = mfma16<TC>;
     using acc_t = typename M::acc_t;
     constexpr int NW = NMAX / 64;
@@ -88,8 +99,13 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             for(int j = 0; j < KC; ++j)
             {
                 const int col = c0 + KC * kc + j;
-                sbuf[j * NMAX + tid]
-                    = (tid < n && col < n) ? TC(Ai[tid + int64_t(col) * ldi]) : TC(0);
+                TC v = TC(0);
+                if(tid < n && col < n)
+                {
+                    const TI a = Ai[tid + int64_t(col) * ldi];
+                    v = rs ? TC(double(a) * rs[tid] * cs[col]) : TC(a);
+                }
+                sbuf[j * NMAX + tid] = v;
             }
             __syncthreads();
 #pragma unroll
@@ -174,7 +190,8 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 if(kc == 0)
                 {
                     // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i
-                    // of CPT right-hand columns; x_k reaches the group by a shuffle
+                    // of CPT right-hand columns; x_k reaches the group through
+                    // v_readlane (both halves of the wave read, each keeps its own)
                     constexpr int CPT = FNB * FNB / NMAX;
                     const int i = tid & (FNB - 1);
                     const int j0 = (tid / FNB) * CPT;
@@ -189,7 +206,9 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 #pragma unroll
                         for(int q = 0; q < CPT; ++q)
                         {
-                            const TC xk = __shfl(x[q], k, FNB);
+                            const TC x_lo = readlane_t(x[q], k);
+                            const TC x_hi = readlane_t(x[q], FNB + k);
+                            const TC xk = (lane & FNB) ? x_hi : x_lo;
                             if(i > k)
                                 x[q] = fma_t(-lik, xk, x[q]);
                         }
@@ -214,23 +233,54 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                     }
                 }
                 // C[rows >= r1] -= L21_t[:, KC*kc : KC*kc+KC] * U_ts[KC*kc : ..., :]
-#pragma unroll 2
-                for(int kk = 0; kk < KC; kk += 4)
+                if constexpr(OPK != MFMA_NATIVE)
                 {
-                    TC bop[2];
+                    // one K = 16 instruction per tile: lane supplies k = kb .. kb+3
+                    const int kb = 4 * (lane >> 4);
+                    float bv[2][4];
 #pragma unroll
                     for(int ct = 0; ct < 2; ++ct)
-                        bop[ct] = sU[(KC * kc + kk + (lane >> 4)) * UB + 16 * ct + (lane & 15)];
+#pragma unroll
+                        for(int e = 0; e < 4; ++e)
+                            bv[ct][e] = sU[(KC * kc + kb + e) * UB + 16 * ct + (lane & 15)];
 #pragma unroll
                     for(int rt = 0; rt < 4; ++rt)
                     {
                         if(64 * w + 16 * rt >= r1) // wave-uniform
                         {
                             const int i = 64 * w + 16 * rt + (lane & 15);
-                            const TC aop = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
+                            const int srow = ssig[t * NMAX + i];
+                            float av[4];
+#pragma unroll
+                            for(int e = 0; e < 4; ++e)
+                                av[e] = -sbuf[(kb + e) * NMAX + srow];
 #pragma unroll
                             for(int ct = 0; ct < 2; ++ct)
-                                acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
+                                acc[rt][ct] = mfma16x16x16<OPK>(av, bv[ct], acc[rt][ct]);
+                        }
+                    }
+                }
+                else
+                {
+#pragma unroll 2
+                    for(int kk = 0; kk < KC; kk += 4)
+                    {
+                        TC bop[2];
+#pragma unroll
+                        for(int ct = 0; ct < 2; ++ct)
+                            bop[ct] = sU[(KC * kc + kk + (lane >> 4)) * UB + 16 * ct + (lane & 15)];
+#pragma unroll
+                        for(int rt = 0; rt < 4; ++rt)
+                        {
+                            if(64 * w + 16 * rt >= r1) // wave-uniform
+                            {
+                                const int i = 64 * w + 16 * rt + (lane & 15);
+                                const TC aop
+                                    = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
+#pragma unroll
+                                for(int ct = 0; ct < 2; ++ct)
+                                    acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
+                            }
                         }
                     }
                 }
@@ -376,37 +426,44 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 // 300-400 registers (1 wave per SIMD); with it the kernel fits 190-240 VGPRs
 // without scratch (a bound of 3 spills), so 2 workgroups of 256 threads
 // share a CU.
-template <typename TI, typename TC, int NMAX>
+template <typename TI, typename TC, int NMAX, int OPK>
 __global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                                         int ldo, int* __restrict__ ipivA,
                                                         int64_t strideP, int* __restrict__ infoA,
-                                                        int bc, const int* __restrict__ skip)
+                                                        int bc, const int* __restrict__ skip,
+                                                        const double* __restrict__ rs,
+                                                        const double* __restrict__ cs,
+                                                        int64_t strideS)
 {
     const int64_t b = blockIdx.x;
     if(skip && skip[b] != 0)
         return;
-    const int zero_at
-        = lu_fused_body<TI, TC, NMAX>(n, AI[b], ldi, AO[b], ldo, ipivA + b * strideP);
+    const int zero_at = lu_fused_body<TI, TC, NMAX, OPK>(
+        n, AI[b], ldi, AO[b], ldo, ipivA + b * strideP, rs ? rs + b * strideS : nullptr,
+        cs ? cs + b * strideS : nullptr);
     if(threadIdx.x == 0)
         infoA[b] = zero_at;
 }
 
-template <typename TI, typename TC>
+template <typename TI, typename TC, int OPK = MFMA_NATIVE>
 rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                int ldo, int* ipiv, int64_t strideP, int* info, int bc,
-                               const int* skip)
+                               const int* skip, const double* rs = nullptr,
+                               const double* cs = nullptr, int64_t strideS = 0)
 {
     if(bc <= 0 || n <= 0)
         return rocblas_status_success;
+    auto go = [&](auto nmax) {
+        constexpr int NMAX = decltype(nmax)::value;
+        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, NMAX, OPK>), dim3(bc), dim3(NMAX), 0, stream,
+                           n, AI, ldi, AO, ldo, ipiv, strideP, info, bc, skip, rs, cs, strideS);
+    };
     if(n <= 128)
-        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 128>), dim3(bc), dim3(128), 0, stream, n, AI,
-                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
+        go(std::integral_constant<int, 128>{});
     else if(n <= 256)
-        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 256>), dim3(bc), dim3(256), 0, stream, n, AI,
-                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
+        go(std::integral_constant<int, 256>{});
     else
-        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, 512>), dim3(bc), dim3(512), 0, stream, n, AI,
-                           ldi, AO, ldo, ipiv, strideP, info, bc, skip);
+        go(std::integral_constant<int, 512>{});
     RSA_LAUNCH_CHECK();
     return rocblas_status_success;
 }

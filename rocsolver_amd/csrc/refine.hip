@@ -19,9 +19,17 @@
 // factors re-read from memory per step) -> FP64 LU and solve of the batch,
 // skipping every system that converged.
 //
-// Right-hand sides are refined in groups of up to 8 columns.  With nrhs <= 8
-// this is exactly dsgesv; with more, each group iterates until its own
-// columns pass the test and ITER is the largest group count.
+// Half-width factorisations (no LAPACK counterpart): A is first equilibrated
+// with power-of-two row and column scales R, C (exact in FP64), the FP32 LU
+// of R A C is computed with FP16 or BF16 matrix-core operands in its trailing
+// updates, and every correction solves (R A C) y = R r, x += C y; residuals
+// and the stopping test use the original A in FP64.  FP32 overflow of A is
+// not a failure there (the scaled matrix is what gets rounded).
+//
+// Right-hand sides are refined in groups of up to 8 columns (4 when n > 256).
+// With nrhs within one group this is exactly dsgesv; with more, each group
+// iterates until its own columns pass the test and ITER is the largest group
+// count.
 #include "internal.hpp"
 #include "trsv_rows.hpp"
 
@@ -44,7 +52,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     int n, int nrhs, Batch<double> AA, int lda, Batch<float> LL, int ldl,
     const int* __restrict__ ipivA, int64_t strideP, const int* __restrict__ info32,
     Batch<double> BB, int ldb, Batch<double> XX, int ldx, int itermax, int* __restrict__ iter,
-    int* __restrict__ info, int* __restrict__ fskip, double* __restrict__ berr)
+    int* __restrict__ info, int* __restrict__ fskip, double* __restrict__ berr,
+    const double* __restrict__ rsA, const double* __restrict__ csA, int64_t strideS)
 {
     constexpr int NW = NMAX / 64;
     // columns of A per load group in the residual (fewer at 512 rows x 8
@@ -55,7 +64,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     __shared__ float sx[2][GBS][NR];
     __shared__ int sipiv[NMAX];
     __shared__ int ssrc[NMAX];
-    __shared__ double sred[NW][2 * NR + 2];
+    __shared__ double sred[NW][2 * NR + 3];
     __shared__ int sflag;
 
     const int64_t b = blockIdx.x;
@@ -67,6 +76,11 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     const double* Bm = BB[b];
     double* X = XX[b];
     const bool row = tid < n;
+    const bool scaled = rsA != nullptr;
+    // this row's equilibration factor and this solution component's (read
+    // where used: fewer live registers)
+    auto rsc = [&]() { return (scaled && row) ? rsA[b * strideS + tid] : 1.0; };
+    auto csc = [&]() { return (scaled && row) ? csA[b * strideS + tid] : 1.0; };
 
     auto finish = [&](int code, bool ok, double err) {
         if(tid == 0)
@@ -94,7 +108,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     // ---- B and A range checks, FP32 LU status (dsgesv order: B, A, LU)
     bool big = false;
     for(int c = 0; c < nrhs; ++c)
-        if(row && beyond_fp32(Bm[tid + int64_t(c) * ldb]))
+        if(row && beyond_fp32(Bm[tid + int64_t(c) * ldb] * rsc()))
             big = true;
     if(any(big))
     {
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     {
         // only then is A scanned here; otherwise the first residual pass does it
         big = false;
-        if(row)
+        if(row && !scaled)
             for(int j = 0; j < n; ++j)
                 if(beyond_fp32(A[tid + int64_t(j) * lda]))
                     big = true;
@@ -124,11 +138,12 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     for(int c0 = 0; c0 < nrhs; c0 += NR)
     {
         const bool measure_a = (c0 == 0);
-        double bv[NR], xv[NR];
+        // B is re-read at every residual instead of held in registers
+        auto bval = [&](int q) {
+            return (row && c0 + q < nrhs) ? Bm[tid + int64_t(c0 + q) * ldb] : 0.0;
+        };
+        double xv[NR];
         float d[NR];
-#pragma unroll
-        for(int q = 0; q < NR; ++q)
-            bv[q] = (row && c0 + q < nrhs) ? Bm[tid + int64_t(c0 + q) * ldb] : 0.0;
 
         // correction solve of the FP32 right-hand side rv (row order) -> d
         auto solve = [&](const float (&rv)[NR]) {
@@ -147,11 +162,12 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             float rv[NR];
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                rv[q] = float(bv[q]);
+                rv[q] = float(bval(q) * rsc());
             solve(rv);
+            const double cj = csc();
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] = double(d[q]);
+                xv[q] = double(d[q]) * cj;
         }
 
         int code = 0; // 0: running, 1: converged, <0: fallback code
@@ -167,7 +183,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             double acc[NR];
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                acc[q] = bv[q];
+                acc[q] = bval(q);
             const bool measure = measure_a && it == 0;
             double rs = 0, amax = 0;
             if(row)
@@ -206,14 +222,19 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                         cur[u] = nxt[u];
                 }
             }
-            // ---- column maxima of |R| and |X| (and ||A||_inf, max |a| once)
-            double rn[NR], xn[NR];
+            // ---- column maxima of |R| and |X| (and ||A||_inf, max |a| once),
+            //      and the largest entry of the right-hand side of the next solve
+            double rn[NR], xn[NR], rsmax = 0;
+            const double ri = rsc();
 #pragma unroll
             for(int q = 0; q < NR; ++q)
             {
                 rn[q] = group_max<64>(row ? fabs(acc[q]) : 0.0);
                 xn[q] = group_max<64>(row ? fabs(xv[q]) : 0.0);
+                if(row && c0 + q < nrhs)
+                    rsmax = fmax(rsmax, fabs(acc[q] * ri));
             }
+            rsmax = group_max<64>(rsmax);
             if(measure)
             {
                 rs = group_max<64>(rs);
@@ -229,8 +250,12 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                 }
                 sred[w][2 * NR] = rs;
                 sred[w][2 * NR + 1] = amax;
+                sred[w][2 * NR + 2] = rsmax;
             }
             __syncthreads();
+            rsmax = sred[0][2 * NR + 2];
+            for(int v = 1; v < NW; ++v)
+                rsmax = fmax(rsmax, sred[v][2 * NR + 2]);
 #pragma unroll
             for(int q = 0; q < NR; ++q)
             {
@@ -251,14 +276,15 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                     am = fmax(am, sred[v][2 * NR + 1]);
                 }
                 anorm = an;
-                if(am > double(FLT_MAX)) // the FP32 copy of A overflowed
+                if(!scaled && am > double(FLT_MAX)) // the FP32 copy of A overflowed
                     code = -2;
             }
             __syncthreads(); // sred is rewritten by the next step
             if(code != 0)
                 break;
             const double cte = anorm * cte_scale;
-            bool conv = true, rbig = false;
+            bool conv = true;
+            const bool rbig = beyond_fp32(rsmax); // dlag2s.f on the next right-hand side
             err_c = 0;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
@@ -266,8 +292,6 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                 {
                     if(rn[q] > xn[q] * cte) // dsgesv.f: IF( RNRM.GT.XNRM*CTE ) not converged
                         conv = false;
-                    if(beyond_fp32(rn[q]))
-                        rbig = true;
                     const double dn = anorm * xn[q];
                     err_c = fmax(err_c, dn > 0 ? rn[q] / dn : (rn[q] > 0 ? double(INFINITY) : 0.0));
                 }
@@ -288,13 +312,15 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             }
             // ---- X += fp64(U^-1 L^-1 P fp32(R))
             float rv[NR];
+            const double ri2 = rsc();
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                rv[q] = float(acc[q]);
+                rv[q] = float(acc[q] * ri2);
             solve(rv);
+            const double cj = csc();
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] += double(d[q]);
+                xv[q] += double(d[q]) * cj;
         }
         if(code < 0)
         {
@@ -312,6 +338,47 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     finish(it_all, true, worst);
 }
 
+// Power of two 2^-e with m * 2^-e in [1/2, 1); 1 for a zero or non-finite m.
+__device__ __forceinline__ double pow2_scale(double m)
+{
+    if(!(m > 0.0) || m > DBL_MAX)
+        return 1.0;
+    int e;
+    frexp(m, &e);
+    return ldexp(1.0, -e);
+}
+
+// Row scales rs_i (largest |a_ij| of row i), then column scales cs_j (largest
+// |rs_i a_ij| of column j), powers of two (the radix-power choice of LAPACK
+// dgeequb, so the scaled entries are exact).  One workgroup per matrix.
+template <int NT>
+__global__ __launch_bounds__(NT) void equilibrate_kernel(int n, Batch<double> AA, int lda,
+                                                         double* __restrict__ rs,
+                                                         double* __restrict__ cs,
+                                                         int64_t strideS)
+{
+    __shared__ double sr[FUSED_MAX];
+    const int64_t b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const double* A = AA[b];
+    for(int i = tid; i < n; i += NT)
+    {
+        double m = 0;
+        for(int j = 0; j < n; ++j)
+            m = fmax(m, fabs(A[i + int64_t(j) * lda]));
+        sr[i] = pow2_scale(m);
+        rs[b * strideS + i] = sr[i];
+    }
+    __syncthreads();
+    for(int j = tid; j < n; j += NT)
+    {
+        double m = 0;
+        for(int i = 0; i < n; ++i)
+            m = fmax(m, fabs(A[i + int64_t(j) * lda]) * sr[i]);
+        cs[b * strideS + j] = pow2_scale(m);
+    }
+}
+
 // after the FP64 fallback LU: no solve for systems whose FP64 LU is singular
 __global__ void ir_mark_singular_kernel(const int* __restrict__ info, int* __restrict__ fskip,
                                         int bc)
@@ -325,25 +392,28 @@ size_t dsgesv_workspace_bytes(int n, int nrhs, int bc, int itermax)
 {
     (void)nrhs;
     (void)itermax;
-    const size_t per = sizeof(double) * size_t(n) * n + sizeof(int) * (size_t(n) + 2);
-    return per * size_t(bc) + 4 * 256;
+    const size_t per
+        = sizeof(double) * (size_t(n) * n + 2 * size_t(n)) + sizeof(int) * (size_t(n) + 2);
+    return per * size_t(bc) + 6 * 256;
 }
 
 template <int NMAX, int NR>
 static void launch_ir(hipStream_t stream, int n, int nrhs, Batch<double> A, int lda,
                       Batch<float> LU, const int* ipiv, const int* info32, Batch<double> B,
                       int ldb, Batch<double> X, int ldx, int itermax, int* iter, int* info,
-                      int* fskip, double* berr, int bc)
+                      int* fskip, double* berr, int bc, const double* rs, const double* cs)
 {
     hipLaunchKernelGGL((ir_fused_kernel<NMAX, NR>), dim3(bc), dim3(NMAX), 0, stream, n, nrhs, A,
                        lda, LU, n, ipiv, int64_t(n), info32, B, ldb, X, ldx, itermax, iter, info,
-                       fskip, berr);
+                       fskip, berr, rs, cs, int64_t(n));
 }
 
-rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> A, int lda,
-                             Batch<double> B, int ldb, Batch<double> X, int ldx, int* iter,
-                             int* info, double* berr, int bc, int itermax, void* work)
+rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Batch<double> A,
+                             int lda, Batch<double> B, int ldb, Batch<double> X, int ldx,
+                             int* iter, int* info, double* berr, int bc, int itermax, void* work)
 {
+    if(factor < 0 || factor > 2)
+        return rocblas_status_invalid_value;
     if(n <= SMALL_MAX || n > FUSED_MAX)
         return rocblas_status_not_implemented;
     if(bc <= 0 || nrhs <= 0)
@@ -359,6 +429,10 @@ rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> 
     int* ipiv = reinterpret_cast<int*>(take(sizeof(int) * size_t(n) * bc));
     int* info32 = reinterpret_cast<int*>(take(sizeof(int) * bc));
     int* fskip = reinterpret_cast<int*>(take(sizeof(int) * bc));
+    double* rs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
+    double* cs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
+    if(factor == 0)
+        rs = cs = nullptr;
 
     const int64_t slot = int64_t(n) * n; // LU slot per matrix, in doubles
     Batch<float> LU32 = strided(reinterpret_cast<float*>(lu), 2 * slot);
@@ -372,18 +446,35 @@ rocblas_status dsgesv_refine(hipStream_t stream, int n, int nrhs, Batch<double> 
             return st;                    \
     } while(0)
 
-    RSA_TRY((lu_fused<double, float>(stream, n, A, lda, LU32, n, ipiv, n, info32, bc)));
+    if(factor == 0)
+        RSA_TRY((lu_fused<double, float>(stream, n, A, lda, LU32, n, ipiv, n, info32, bc)));
+    else
+    {
+        if(n <= 128)
+            hipLaunchKernelGGL((equilibrate_kernel<128>), dim3(bc), dim3(128), 0, stream, n, A,
+                               lda, rs, cs, int64_t(n));
+        else
+            hipLaunchKernelGGL((equilibrate_kernel<256>), dim3(bc), dim3(256), 0, stream, n, A,
+                               lda, rs, cs, int64_t(n));
+        RSA_LAUNCH_CHECK();
+        RSA_TRY(lu_fused_lowp(stream, factor, n, A, lda, LU32, n, ipiv, n, info32, bc, rs, cs,
+                              int64_t(n)));
+    }
     auto by_nr = [&](auto nmax) {
         constexpr int NMAX = decltype(nmax)::value;
         if(nrhs <= 1)
             launch_ir<NMAX, 1>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc);
+                               itermax, iter, info, fskip, berr, bc, rs, cs);
         else if(nrhs <= 4)
             launch_ir<NMAX, 4>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc);
+                               itermax, iter, info, fskip, berr, bc, rs, cs);
+        else if(NMAX < 512) // groups of 4 beyond 256 rows (8 would spill registers)
+            launch_ir<(NMAX < 512 ? NMAX : 256), 8>(stream, n, nrhs, A, lda, LU32, ipiv, info32,
+                                                    B, ldb, X, ldx, itermax, iter, info, fskip,
+                                                    berr, bc, rs, cs);
         else
-            launch_ir<NMAX, 8>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc);
+            launch_ir<NMAX, 4>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
+                               itermax, iter, info, fskip, berr, bc, rs, cs);
     };
     if(n <= 128)
         by_nr(std::integral_constant<int, 128>{});

@@ -26,17 +26,6 @@ namespace rsa
 
 constexpr int GBS = 32; // diagonal block
 
-__device__ __forceinline__ float readlane_t(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double readlane_t(double v, int l)
-{
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
-    return __hiloint2double(hi, lo);
-}
-
 // Entries c0 .. c0+31 of row i of op(A) restricted to one block column:
 // op N reads A[i, c] (coalesced across threads), op T reads A[c, i] (32
 // consecutive entries per thread).  Entries with c >= n are zero; inactive

@@ -70,3 +70,62 @@ def test_dsgesv_fallback_paths(torch_gpu, handle):
     ok = info == 0
     be = oracle.solve_backward_error(A[ok], n, X[ok], B[ok])
     assert be.max() <= 10 * n * EPS64
+
+
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+@pytest.mark.parametrize("n,nrhs,bc", [(96, 4, 6), (256, 8, 4), (300, 3, 2)])
+def test_half_width_factor_refines_to_fp64(torch_gpu, handle, factor, n, nrhs, bc):
+    """Equilibrated FP16/BF16-operand LU + FP64 refinement: well-conditioned
+    systems converge by refinement to FP64 backward error; the result agrees
+    with the oracle's FP64 solution within kappa * n * eps64."""
+    print(f"dxgesv {factor} n={n} nrhs={nrhs} bc={bc}", flush=True)
+    g = np.random.default_rng(n + nrhs)
+    # kappa ~ 10 with different row and column scales (2^-10 .. 2^10), which
+    # the equilibration removes exactly
+    A = g.standard_normal((bc, n, n)) + 2.0 * np.sqrt(n) * np.eye(n)[None]
+    A *= np.exp2(g.integers(-10, 11, size=(bc, n, 1)))
+    A *= np.exp2(g.integers(-10, 11, size=(bc, 1, n)))
+    B = g.standard_normal((bc, nrhs, n))
+    Xo, _, info_o = oracle.gesv(A.copy(), n, B.copy())
+    X, it, info, berr = ra.dsgesv(handle, dev(torch_gpu, A), dev(torch_gpu, B), factor=factor)
+    torch_gpu.cuda.synchronize()
+    X, it, info = X.cpu().numpy(), it.cpu().numpy(), info.cpu().numpy()
+    assert np.all(info == 0) and np.all(info_o == 0)
+    assert np.all(it >= 1), it  # refined (a half-width LU alone is not FP64 accurate)
+    assert np.all(it <= 20), it
+    be = oracle.solve_backward_error(A, n, X, B)
+    assert be.max() <= 10 * n * EPS64, be
+    fe = np.abs(X - Xo).max(axis=(1, 2)) / np.abs(Xo).max(axis=(1, 2))
+    assert fe.max() <= 1e3 * n * EPS64, fe
+
+
+def test_half_width_factor_fallback_and_fp32_identity(torch_gpu, handle):
+    g = np.random.default_rng(21)
+    n, nrhs, bc = 128, 2, 3
+    A = g.standard_normal((bc, n, n)) + 3 * np.sqrt(n) * np.eye(n)[None]
+    U, _ = np.linalg.qr(g.standard_normal((n, n)))
+    V, _ = np.linalg.qr(g.standard_normal((n, n)))
+    A[1] = ((U * np.logspace(0, -9, n)) @ V.T).T  # kappa 1e9: FP16 refinement cannot converge
+    A[2] *= 1e300  # beyond FP32 range: factor 0 gives -2, the scaled factors do not
+    B = g.standard_normal((bc, nrhs, n))
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    X16, it16, info16, _ = ra.dsgesv(handle, Ad, Bd, factor="fp16")
+    X32, it32, info32, b32 = ra.dsgesv(handle, Ad, Bd, factor="fp32")
+    torch_gpu.cuda.synchronize()
+    it16, it32 = it16.cpu().numpy(), it32.cpu().numpy()
+    assert it16[0] >= 1 and it16[1] == -31 and it16[2] >= 1
+    assert it32[2] == -2
+    assert np.all(info16.cpu().numpy() == 0)
+    be = oracle.solve_backward_error(A, n, X16.cpu().numpy(), B)
+    assert be.max() <= 10 * n * EPS64, be
+    # factor 0 through the general entry point is the dsgesv entry point
+    X0 = torch_gpu.zeros_like(X32)
+    it0 = torch_gpu.zeros(bc, dtype=torch_gpu.int32, device="cuda")
+    info0 = torch_gpu.zeros(bc, dtype=torch_gpu.int32, device="cuda")
+    b0 = torch_gpu.zeros(bc, dtype=torch_gpu.float64, device="cuda")
+    ra.check("rocsolver_amd_dxgesv_strided_batched", handle.value, 0, n, nrhs, Ad.data_ptr(), n,
+             n * n, Bd.data_ptr(), n, n * nrhs, X0.data_ptr(), n, n * nrhs, it0.data_ptr(),
+             info0.data_ptr(), b0.data_ptr(), bc)
+    torch_gpu.cuda.synchronize()
+    assert torch_gpu.equal(X0, X32)
+    np.testing.assert_array_equal(it0.cpu().numpy(), it32)
