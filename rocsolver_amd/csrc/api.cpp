@@ -281,6 +281,10 @@ rocblas_status vbatched_impl(rocblas_handle handle, int op, rocblas_operation tr
 {
     if(!handle)
         return rocblas_status_invalid_handle;
+    // getrf / getrs alone are switched off: their first GPU test run ended in a
+    // failure followed by a hang whose cause has not been found; gesv is on
+    if(op != 2)
+        return rocblas_status_not_implemented;
     bool tr = false;
     if(op == 1)
     {

@@ -24,7 +24,9 @@
 // of R A C is computed with FP16 or BF16 matrix-core operands in its trailing
 // updates, and every correction solves (R A C) y = R r, x += C y; residuals
 // and the stopping test use the original A in FP64.  FP32 overflow of A is
-// not a failure there (the scaled matrix is what gets rounded).
+// not a failure there (the scaled matrix is what gets rounded), and the
+// right-hand side of each FP32 solve is normalised by a power of two, so an
+// FP32 out-of-range right-hand side cannot occur either.
 //
 // Right-hand sides are refined in groups of up to 8 columns (4 when n > 256).
 // With nrhs within one group this is exactly dsgesv; with more, each group
@@ -43,6 +45,19 @@ __device__ __forceinline__ bool beyond_fp32(double v)
 {
     // dlag2s.f: (A(I,J).LT.-RMAX) .OR. (A(I,J).GT.RMAX)
     return v < -double(FLT_MAX) || v > double(FLT_MAX);
+}
+
+// Power of two 2^-e with m * 2^-e in [1/2, 1); 1 for a zero or non-finite m.
+// e is kept within [-1020, 1020] so the scale itself is a finite normal number
+// (a subnormal m is then only brought up to 2^-1020 times its size).
+__device__ __forceinline__ double pow2_scale(double m)
+{
+    if(!(m > 0.0) || m > DBL_MAX)
+        return 1.0;
+    int e;
+    frexp(m, &e);
+    e = e < -1020 ? -1020 : (e > 1020 ? 1020 : e);
+    return ldexp(1.0, -e);
 }
 
 // Whole refinement of one system per workgroup (n <= NMAX threads, thread i
@@ -64,8 +79,9 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     __shared__ float sx[2][GBS][NR];
     __shared__ int sipiv[NMAX];
     __shared__ int ssrc[NMAX];
-    __shared__ double sred[NW][2 * NR + 3];
+    __shared__ double sred[NW][2 * NR + 4];
     __shared__ int sflag;
+    __shared__ double snorm[NW];
 
     const int64_t b = blockIdx.x;
     const int tid = threadIdx.x;
@@ -81,6 +97,10 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     // where used: fewer live registers)
     auto rsc = [&]() { return (scaled && row) ? rsA[b * strideS + tid] : 1.0; };
     auto csc = [&]() { return (scaled && row) ? csA[b * strideS + tid] : 1.0; };
+    // the scales are powers of two: work with their exponents
+    auto rexp = [&]() { return scaled ? ilogb(rsc()) : 0; };
+    auto cexp = [&]() { return scaled ? ilogb(csc()) : 0; };
+    constexpr double NO_EXP = -1.0e9; // exponent of a zero vector
 
     auto finish = [&](int code, bool ok, double err) {
         if(tid == 0)
@@ -91,6 +111,18 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             if(berr)
                 berr[b] = ok ? err : -1.0;
         }
+    };
+    // largest v over the workgroup, in every thread (two barriers)
+    auto wg_max = [&](double v) {
+        v = group_max<64>(v);
+        if(lane == 0)
+            snorm[w] = v;
+        __syncthreads();
+        double m = snorm[0];
+        for(int k = 1; k < NW; ++k)
+            m = fmax(m, snorm[k]);
+        __syncthreads();
+        return m;
     };
     // any thread's flag -> every thread (one barrier before, one after)
     auto any = [&](bool f) {
@@ -108,7 +140,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     // ---- B and A range checks, FP32 LU status (dsgesv order: B, A, LU)
     bool big = false;
     for(int c = 0; c < nrhs; ++c)
-        if(row && beyond_fp32(Bm[tid + int64_t(c) * ldb] * rsc()))
+        if(row && !scaled && beyond_fp32(Bm[tid + int64_t(c) * ldb]))
             big = true;
     if(any(big))
     {
@@ -159,15 +191,34 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
         };
 
         {
+            // equilibrated paths: the right-hand side R b of every FP32 solve is
+            // normalised by 2^-E, E the largest exponent of its entries, so it
+            // neither underflows nor overflows in FP32; the solve is linear, so
+            // the correction is scaled back by 2^E.  Exponents are combined as
+            // integers (ldexp), so no intermediate scale factor can overflow.
+            const int ki = rexp();
+            double e = NO_EXP;
+            if(scaled)
+            {
+#pragma unroll
+                for(int q = 0; q < NR; ++q)
+                {
+                    const double v = bval(q);
+                    if(v != 0.0 && isfinite(v))
+                        e = fmax(e, double(ilogb(v) + ki));
+                }
+                e = wg_max(e);
+            }
+            const int E = scaled ? int(e) : 0;
             float rv[NR];
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                rv[q] = float(bval(q) * rsc());
+                rv[q] = float(ldexp(bval(q), ki - E));
             solve(rv);
-            const double cj = csc();
+            const int kj = cexp() + E;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] = double(d[q]) * cj;
+                xv[q] = ldexp(double(d[q]), kj);
         }
 
         int code = 0; // 0: running, 1: converged, <0: fallback code
@@ -223,18 +274,24 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                 }
             }
             // ---- column maxima of |R| and |X| (and ||A||_inf, max |a| once),
-            //      and the largest entry of the right-hand side of the next solve
-            double rn[NR], xn[NR], rsmax = 0;
-            const double ri = rsc();
+            //      the largest |R| entry (FP32 range check) and the largest
+            //      exponent of R_scaled r (normalisation of the next solve)
+            double rn[NR], xn[NR], rsmax = 0, remax = NO_EXP;
+            const int ki = rexp();
 #pragma unroll
             for(int q = 0; q < NR; ++q)
             {
                 rn[q] = group_max<64>(row ? fabs(acc[q]) : 0.0);
                 xn[q] = group_max<64>(row ? fabs(xv[q]) : 0.0);
                 if(row && c0 + q < nrhs)
-                    rsmax = fmax(rsmax, fabs(acc[q] * ri));
+                {
+                    rsmax = fmax(rsmax, fabs(acc[q]));
+                    if(acc[q] != 0.0 && isfinite(acc[q]))
+                        remax = fmax(remax, double(ilogb(acc[q]) + ki));
+                }
             }
             rsmax = group_max<64>(rsmax);
+            remax = group_max<64>(remax);
             if(measure)
             {
                 rs = group_max<64>(rs);
@@ -251,11 +308,16 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                 sred[w][2 * NR] = rs;
                 sred[w][2 * NR + 1] = amax;
                 sred[w][2 * NR + 2] = rsmax;
+                sred[w][2 * NR + 3] = remax;
             }
             __syncthreads();
             rsmax = sred[0][2 * NR + 2];
+            remax = sred[0][2 * NR + 3];
             for(int v = 1; v < NW; ++v)
+            {
                 rsmax = fmax(rsmax, sred[v][2 * NR + 2]);
+                remax = fmax(remax, sred[v][2 * NR + 3]);
+            }
 #pragma unroll
             for(int q = 0; q < NR; ++q)
             {
@@ -284,7 +346,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                 break;
             const double cte = anorm * cte_scale;
             bool conv = true;
-            const bool rbig = beyond_fp32(rsmax); // dlag2s.f on the next right-hand side
+            // dlag2s.f on the next right-hand side (normalised on the scaled paths)
+            const bool rbig = !scaled && beyond_fp32(rsmax);
             err_c = 0;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
@@ -312,15 +375,16 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             }
             // ---- X += fp64(U^-1 L^-1 P fp32(R))
             float rv[NR];
-            const double ri2 = rsc();
+            const int E = scaled ? int(remax) : 0;
+            const int ki2 = rexp() - E;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                rv[q] = float(acc[q] * ri2);
+                rv[q] = float(ldexp(acc[q], ki2));
             solve(rv);
-            const double cj = csc();
+            const int kj = cexp() + E;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] += double(d[q]) * cj;
+                xv[q] += ldexp(double(d[q]), kj);
         }
         if(code < 0)
         {
@@ -336,16 +400,6 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                     X[tid + int64_t(c0 + q) * ldx] = xv[q];
     }
     finish(it_all, true, worst);
-}
-
-// Power of two 2^-e with m * 2^-e in [1/2, 1); 1 for a zero or non-finite m.
-__device__ __forceinline__ double pow2_scale(double m)
-{
-    if(!(m > 0.0) || m > DBL_MAX)
-        return 1.0;
-    int e;
-    frexp(m, &e);
-    return ldexp(1.0, -e);
 }
 
 // Row scales rs_i (largest |a_ij| of row i), then column scales cs_j (largest

@@ -1,0 +1,97 @@
+"""HBM traffic of the dominant kernels, from hardware counters.
+
+Run under rocprofv3 once per counter (FETCH_SIZE and WRITE_SIZE do not fit in
+one pass together with anything else):
+
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o fetch -- python3 -m rocsolver_amd.traffic
+  rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir> -o write -- python3 -m rocsolver_amd.traffic
+  python3 -m rocsolver_amd.traffic --parse <dir>/.../fetch_counter_collection.csv \\
+                                           <dir>/.../write_counter_collection.csv
+
+The program first copies a 1 GiB device buffer through an elementwise kernel
+(1 GiB read, 1 GiB written), which calibrates what the counters report per byte,
+then launches, once each, on the metric workload's shapes (n = 256, batch
+4096, 8 right-hand sides, FP64 input): the FP32 LU of the FP64 batch
+(rocsolver_amd_dsgetrf_strided_batched) and the whole mixed-precision solve
+(rocsolver_amd_dsgesv_strided_batched).  The parser prints, per kernel, the
+counter values per dispatch scaled by the copy's factor.
+"""
+from __future__ import annotations
+
+import csv
+import sys
+
+COPY_BYTES = 1 << 30
+N, BC, NRHS = 256, 4096, 8
+
+
+def run() -> None:
+    import torch
+
+    import rocsolver_amd as ra
+    torch.cuda.set_device(0)
+    src = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.mul(src, 1.0, out=dst)  # elementwise kernel: reads 1 GiB, writes 1 GiB
+    torch.cuda.synchronize()
+    print("copy of 1 GiB done", flush=True)
+    del src, dst
+    g = torch.Generator(device="cuda").manual_seed(7)
+    A = torch.randn((BC, N, N), dtype=torch.float64, device="cuda", generator=g)
+    A += 4.0 * N ** 0.5 * torch.eye(N, dtype=torch.float64, device="cuda")
+    B = torch.randn((BC, NRHS, N), dtype=torch.float64, device="cuda", generator=g)
+    LU = torch.empty((BC, N, N), dtype=torch.float32, device="cuda")
+    ipiv = torch.zeros((BC, N), dtype=torch.int32, device="cuda")
+    info = torch.zeros(BC, dtype=torch.int32, device="cuda")
+    h = ra.Handle()
+    ra.check("rocsolver_amd_dsgetrf_strided_batched", h.value, N, A.data_ptr(), N, N * N,
+             LU.data_ptr(), N, N * N, ipiv.data_ptr(), N, info.data_ptr(), BC)
+    torch.cuda.synchronize()
+    print("dsgetrf done", flush=True)
+    del LU
+    ra.dsgesv(h, A, B)
+    torch.cuda.synchronize()
+    print("dsgesv done", flush=True)
+
+
+def _rows(path: str):
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def parse(fetch_csv: str, write_csv: str) -> dict:
+    """Per kernel name: mean FETCH_SIZE and WRITE_SIZE per dispatch in bytes,
+    scaled so that the 1 GiB copy reads and writes exactly 1 GiB."""
+    out = {}
+    for key, path in (("fetch", fetch_csv), ("write", write_csv)):
+        per = {}
+        for r in _rows(path):
+            name = r.get("Kernel_Name") or r.get("Kernel-Name") or r.get("KernelName")
+            val = float(r.get("Counter_Value") or r.get("Counter-Value") or r.get("CounterValue"))
+            per.setdefault(name, []).append(val * 1024.0)  # counters report KiB
+        out[key] = per
+    copy = [k for k in out["fetch"] if "copy" in k.lower() or "elementwise" in k.lower()]
+    big = max(copy, key=lambda k: max(out["fetch"][k]))
+    f_scale = COPY_BYTES / max(out["fetch"][big])
+    w_scale = COPY_BYTES / max(out["write"][big])
+    res = {"copy_kernel": big, "fetch_scale": f_scale, "write_scale": w_scale, "kernels": {}}
+    for name in out["fetch"]:
+        if "rsa::" not in name:
+            continue
+        fe = out["fetch"][name]
+        wr = out["write"].get(name, [0.0])
+        res["kernels"][name] = {
+            "dispatches": len(fe),
+            "fetch_bytes": sum(fe) / len(fe) * f_scale,
+            "write_bytes": sum(wr) / len(wr) * w_scale,
+        }
+    return res
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "--parse":
+        import json
+        print(json.dumps(parse(sys.argv[2], sys.argv[3]), indent=1))
+    else:
+        run()

@@ -83,10 +83,12 @@ def test_half_width_factor_refines_to_fp64(torch_gpu, handle, factor, n, nrhs, b
     # kappa ~ 10 with different row and column scales (2^-10 .. 2^10), which
     # the equilibration removes exactly
     A = g.standard_normal((bc, n, n)) + 2.0 * np.sqrt(n) * np.eye(n)[None]
-    A *= np.exp2(g.integers(-10, 11, size=(bc, n, 1)))
-    A *= np.exp2(g.integers(-10, 11, size=(bc, 1, n)))
+    dc = np.exp2(g.integers(-10, 11, size=(bc, n, 1)))  # column scales (A is stored transposed)
+    dr = np.exp2(g.integers(-10, 11, size=(bc, 1, n)))  # row scales
+    A *= dc * dr
     B = g.standard_normal((bc, nrhs, n))
-    Xo, _, info_o = oracle.gesv(A.copy(), n, B.copy())
+    Xo = B.copy()
+    _, info_o = oracle.gesv(A.copy(), n, Xo)
     X, it, info, berr = ra.dsgesv(handle, dev(torch_gpu, A), dev(torch_gpu, B), factor=factor)
     torch_gpu.cuda.synchronize()
     X, it, info = X.cpu().numpy(), it.cpu().numpy(), info.cpu().numpy()
@@ -95,8 +97,10 @@ def test_half_width_factor_refines_to_fp64(torch_gpu, handle, factor, n, nrhs, b
     assert np.all(it <= 20), it
     be = oracle.solve_backward_error(A, n, X, B)
     assert be.max() <= 10 * n * EPS64, be
-    fe = np.abs(X - Xo).max(axis=(1, 2)) / np.abs(Xo).max(axis=(1, 2))
-    assert fe.max() <= 1e3 * n * EPS64, fe
+    # (no forward-error check here: with rows and columns scaled by 2^+-10 a
+    # normwise-small residual -- what refinement and FP64 GEPP both deliver --
+    # does not determine x to better than kappa(A) * eps ~ 2^40 * eps; the
+    # well-scaled case below checks the forward error)
 
 
 def test_half_width_factor_fallback_and_fp32_identity(torch_gpu, handle):
@@ -118,6 +122,11 @@ def test_half_width_factor_fallback_and_fp32_identity(torch_gpu, handle):
     assert np.all(info16.cpu().numpy() == 0)
     be = oracle.solve_backward_error(A, n, X16.cpu().numpy(), B)
     assert be.max() <= 10 * n * EPS64, be
+    # well-scaled, well-conditioned entry 0: forward error against the FP64 oracle
+    Xo = B[:1].copy()
+    oracle.gesv(A[:1].copy(), n, Xo)
+    fe = np.abs(X16.cpu().numpy()[0] - Xo[0]).max() / np.abs(Xo[0]).max()
+    assert fe <= 1e3 * n * EPS64, fe
     # factor 0 through the general entry point is the dsgesv entry point
     X0 = torch_gpu.zeros_like(X32)
     it0 = torch_gpu.zeros(bc, dtype=torch_gpu.int32, device="cuda")
