@@ -106,7 +106,7 @@ rocblas_status gesv_adjoint(hipStream_t stream, int n, int nrhs, Batch<T> LU, in
         rocblas_status st;
         if(n <= SMALL_MAX)
             st = getrs_small<T>(stream, true, n, nrhs, LU, lda, ipiv, strideP, dB, lddb, bc, true);
-        else if(n <= FUSED_MAX && fits_buffer<T>(lda, n))
+        else if(n <= FUSED_MAX && fits_buffer<T>(lda, n) && fits_buffer<T>(lddb, nrhs))
             st = getrs_fused<T, T>(stream, true, n, nrhs, LU, lda, ipiv, strideP, dB, lddb, dB,
                                    lddb, bc, nullptr, false);
         else

@@ -90,7 +90,7 @@ rocblas_status getrs_impl(rocblas_handle handle, rocblas_operation trans, int n,
     rocblas_get_stream(handle, &stream);
     if(n <= SMALL_MAX)
         return getrs_small<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true);
-    if(n <= FUSED_MAX && fits_buffer<T>(lda, n))
+    if(n <= FUSED_MAX && fits_buffer<T>(lda, n) && fits_buffer<T>(ldb, nrhs))
         return getrs_fused<T, T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb, bc,
                                  nullptr, false);
     return getrs_blocked<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
@@ -121,7 +121,7 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
         return reset_info(stream, info, bc);
     if(n <= SMALL_MAX)
         return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
-    if(n <= FUSED_MAX && fits_buffer<T>(lda, n))
+    if(n <= FUSED_MAX && fits_buffer<T>(lda, n) && fits_buffer<T>(ldb, nrhs))
     {
         rocblas_status st = lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
         if(st != rocblas_status_success)
@@ -187,7 +187,8 @@ rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, c
         rocblas_status st = reset_info(stream, info, bc);
         return st != rocblas_status_success ? st : reset_info(stream, iter, bc);
     }
-    if(n <= SMALL_MAX || n > FUSED_MAX || !fits_buffer<double>(lda, n))
+    if(n <= SMALL_MAX || n > FUSED_MAX || !fits_buffer<double>(lda, n)
+       || !fits_buffer<double>(ldb, nrhs))
         return rocblas_status_not_implemented;
     rocblas_device_malloc mem(handle, ws);
     if(!mem)

@@ -30,23 +30,31 @@ __device__ __forceinline__ void getrs_fused_body(int n, int nrhs, const TL* __re
                                                  const TB* Bm, int ldb, TB* X, int ldx)
 {
     __shared__ TL sx[2][GBS][NR];
+    __shared__ TL sL[GBS][GBS + 1];
     __shared__ int sipiv[NMAX];
     __shared__ int ssrc[NMAX];
 
     const int tid = threadIdx.x;
     const MatrixBuffer<TL> A(Ap, lda, n);
+    const MatrixBuffer<TB> Bbuf(Bm, ldb, nrhs);
     const int pos = pivot_position(n, ipiv, TRANS, sipiv, ssrc);
     const int src = TRANS ? tid : ssrc[tid];
     const int dst = TRANS ? pos : tid;
 
     for(int c0 = 0; c0 < nrhs; c0 += NR)
     {
+        // all NR loads issued back to back (out-of-range offsets read 0)
         TL x[NR];
 #pragma unroll
         for(int q = 0; q < NR; ++q)
-            x[q] = (tid < n && c0 + q < nrhs) ? TL(Bm[src + int64_t(c0 + q) * ldb]) : TL(0);
-        solve_lower_rows<TL, NR, TRANS>(n, A, x, sx);
-        solve_upper_rows<TL, NR, TRANS>(n, A, x, sx);
+        {
+            const uint32_t off = (tid < n && c0 + q < nrhs)
+                                     ? Bbuf.row_offset(src) + Bbuf.col_offset(c0 + q)
+                                     : MatrixBuffer<TB>::OUT_OF_RANGE;
+            x[q] = TL(Bbuf.load(off, 0));
+        }
+        solve_lower_rows<TL, NR, TRANS>(n, A, x, sx, sL);
+        solve_upper_rows<TL, NR, TRANS>(n, A, x, sx, sL);
         // every thread has read its right-hand side before any row is stored
         // (X may alias B and op = T stores at the permuted row)
         __syncthreads();

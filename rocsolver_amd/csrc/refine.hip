@@ -77,6 +77,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     __shared__ double sX[NMAX][NR];
     __shared__ float sR[NMAX][NR];
     __shared__ float sx[2][GBS][NR];
+    __shared__ float sL[GBS][GBS + 1];
     __shared__ int sipiv[NMAX];
     __shared__ int ssrc[NMAX];
     __shared__ double sred[NW][2 * NR + 4];
@@ -92,11 +93,18 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     const double* Bm = BB[b];
     double* X = XX[b];
     const bool row = tid < n;
+    const MatrixBuffer<double> Bbuf(Bm, ldb, nrhs);
+    // B[tid, c] (0 for rows past n or columns past nrhs), branch-free
+    auto bload = [&](int c) {
+        return Bbuf.load((row && c < nrhs) ? Bbuf.row_offset(tid) + Bbuf.col_offset(c)
+                                           : MatrixBuffer<double>::OUT_OF_RANGE,
+                         0);
+    };
     const bool scaled = rsA != nullptr;
     // this row's equilibration factor and this solution component's (read
     // where used: fewer live registers)
-    auto rsc = [&]() { return (scaled && row) ? rsA[b * strideS + tid] : 1.0; };
-    auto csc = [&]() { return (scaled && row) ? csA[b * strideS + tid] : 1.0; };
+    auto rsc = [&]() { return scaled ? rsA[b * strideS + (row ? tid : 0)] : 1.0; };
+    auto csc = [&]() { return scaled ? csA[b * strideS + (row ? tid : 0)] : 1.0; };
     // the scales are powers of two: work with their exponents
     auto rexp = [&]() { return scaled ? ilogb(rsc()) : 0; };
     auto cexp = [&]() { return scaled ? ilogb(csc()) : 0; };
@@ -139,9 +147,12 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
 
     // ---- B and A range checks, FP32 LU status (dsgesv order: B, A, LU)
     bool big = false;
-    for(int c = 0; c < nrhs; ++c)
-        if(row && !scaled && beyond_fp32(Bm[tid + int64_t(c) * ldb]))
-            big = true;
+    if(!scaled)
+    {
+#pragma unroll 8
+        for(int c = 0; c < nrhs; ++c)
+            big = big || beyond_fp32(bload(c));
+    }
     if(any(big))
     {
         finish(-2, false, 0);
@@ -171,9 +182,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     {
         const bool measure_a = (c0 == 0);
         // B is re-read at every residual instead of held in registers
-        auto bval = [&](int q) {
-            return (row && c0 + q < nrhs) ? Bm[tid + int64_t(c0 + q) * ldb] : 0.0;
-        };
+        auto bval = [&](int q) { return bload(c0 + q); };
         double xv[NR];
         float d[NR];
 
@@ -186,8 +195,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 d[q] = row ? sR[src][q] : 0.f;
-            solve_lower_rows<float, NR, false>(n, LU, d, sx);
-            solve_upper_rows<float, NR, false>(n, LU, d, sx);
+            solve_lower_rows<float, NR, false>(n, LU, d, sx, sL);
+            solve_upper_rows<float, NR, false>(n, LU, d, sx, sL);
         };
 
         {

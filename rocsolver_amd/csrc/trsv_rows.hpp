@@ -2,15 +2,16 @@
 // system per workgroup, thread i owning row i of NR right-hand sides in
 // registers (n <= NMAX threads).
 //
-// Each triangle is walked in 32-row diagonal blocks (one barrier per block):
+// Each triangle is walked in 32-row diagonal blocks (two barriers per block):
 //   * every thread whose row meets the block column loads its 32 entries of
 //     that block column into registers one block ahead of use, so the loads
 //     of block k+1 are in flight while block k is solved;
-//   * the half wave owning the diagonal block solves it in registers: the
-//     value of row k reaches the other 31 lanes through v_readlane (no LDS,
-//     no barrier), and publishes the block's solution to LDS;
-//   * after the barrier every row outside the block subtracts its 32-entry
-//     segment times the block's solution.
+//   * the half wave owning the diagonal block publishes its right-hand sides
+//     and its rows of the block to LDS; every half wave of the workgroup then
+//     solves one right-hand side of the block (a 32-step chain in which the
+//     value of row k reaches the other lanes through v_readlane);
+//   * after the second barrier every row outside the block subtracts its
+//     32-entry segment times the block's solution.
 // The arithmetic is the column-oriented substitution of the reference
 // (library/src/lapack/roclapack_getrs.hpp:117-150 applies laswp, then the
 // unit-lower L, then the non-unit upper U for op = N; U^T, L^T, then the
@@ -29,28 +30,90 @@ constexpr int GBS = 32; // diagonal block
 // Entries c0 .. c0+31 of row i of op(A) restricted to one block column:
 // op N reads A[i, c] (coalesced across threads), op T reads A[c, i] (32
 // consecutive entries per thread).  Entries with c >= n are zero; inactive
-// rows load nothing (out-of-range buffer offset) and get zeros.
+// rows load nothing and get zeros.  For op N the whole byte offset is the
+// per-lane buffer offset, so a column c >= n lies beyond the matrix and the
+// hardware returns 0 without a memory access (no branch, no select, and no
+// scalar register per column); an inactive lane uses an offset beyond the
+// buffer for every column.
 template <typename TL, bool TRANS>
 __device__ __forceinline__ void load_segment(const MatrixBuffer<TL>& A, int i, int c0, int n,
                                              bool active, TL (&v)[GBS])
 {
-    const uint32_t vrow = TRANS ? A.col_offset(i) : A.row_offset(i);
-    const uint32_t voff = active ? vrow : MatrixBuffer<TL>::OUT_OF_RANGE;
-#pragma unroll
-    for(int k = 0; k < GBS; ++k)
+    if(TRANS)
     {
-        const int c = (c0 + k < n) ? c0 + k : n - 1; // uniform
-        const uint32_t soff = TRANS ? A.row_offset(c) : A.col_offset(c);
-        const TL x = A.load(voff, soff);
-        v[k] = (c0 + k < n) ? x : TL(0);
+        const uint32_t vrow = active ? A.col_offset(i) : MatrixBuffer<TL>::OUT_OF_RANGE;
+#pragma unroll
+        for(int k = 0; k < GBS; ++k)
+        {
+            const int c = (c0 + k < n) ? c0 + k : n - 1; // uniform
+            const TL x = A.load(vrow, A.row_offset(c));
+            v[k] = (c0 + k < n) ? x : TL(0);
+        }
+    }
+    else
+    {
+        // running offset (one register): an inactive lane stays beyond the
+        // buffer.  The empty asm hides the increment's value from the
+        // optimiser, which would otherwise keep all 32 column offsets live in
+        // registers across the whole solve.
+        uint32_t off = active ? A.row_offset(i) + A.col_offset(c0) : MatrixBuffer<TL>::OUT_OF_RANGE;
+        const uint32_t step = active ? A.ld_bytes : 0u;
+#pragma unroll
+        for(int k = 0; k < GBS; ++k)
+        {
+            v[k] = A.load(off, 0);
+            off += step;
+            asm volatile("" : "+v"(off));
+        }
+    }
+}
+
+// Solve of one 32-row diagonal block, spread over the workgroup: the
+// owning half wave has published the block's right-hand sides (sx[i][q],
+// rows r0+i) and its rows of the block (sL[i][k] = op(A)[r0+i, r0+k]); half
+// wave h of the workgroup solves right-hand sides q = h, h + H, ... (H half
+// waves), lane i holding row r0+i, the value of row k reaching the other 31
+// lanes through v_readlane (both halves of a wavefront read, each keeps its
+// own).  LOWER: forward over k (unit diagonal unless NONUNIT); else backward.
+// The block's solution replaces sx.  No barrier inside; the caller brackets
+// it with two.
+template <typename TL, int NR, bool LOWER, bool NONUNIT>
+__device__ __forceinline__ void solve_diag_block(int r0, int n, TL (*sx)[NR],
+                                                 const TL (*sL)[GBS + 1])
+{
+    const int tid = threadIdx.x;
+    const int i = tid & (GBS - 1);
+    const int h = tid >> 5;
+    const int H = blockDim.x >> 5;
+    for(int q = h; q < NR; q += H) // q differs between the two halves of a wave
+    {
+        TL y = sx[i][q];
+#pragma unroll
+        for(int kk = 0; kk < GBS; ++kk)
+        {
+            const int k = LOWER ? kk : GBS - 1 - kk;
+            if(r0 + k < n) // uniform
+            {
+                const TL lik = sL[i][k];
+                if(NONUNIT && i == k)
+                    y = y / lik;
+                const TL y_lo = readlane_t(y, k);
+                const TL y_hi = readlane_t(y, GBS + k);
+                const TL yk = (tid & GBS) ? y_hi : y_lo;
+                if(LOWER ? (i > k) : (i < k))
+                    y = fma_t(-lik, yk, y);
+            }
+        }
+        sx[i][q] = y;
     }
 }
 
 // Forward substitution with the lower triangle of op(A): unit diagonal for
-// op N (L), non-unit for op T (U^T).  sx: LDS [2][GBS][NR].
+// op N (L), non-unit for op T (U^T).  sx: LDS [2][GBS][NR]; sL: LDS
+// [GBS][GBS+1].
 template <typename TL, int NR, bool TRANS>
 __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
-                                                 TL (*sx)[GBS][NR])
+                                                 TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
     const int tid = threadIdx.x;
     const int lane32 = tid & (GBS - 1);
@@ -63,33 +126,26 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
         const int par = blk & 1;
         if(blk + 1 < nblk) // rows at or below the next block meet its block column
             load_segment<TL, TRANS>(A, tid, r0 + GBS, n, tid >= r0 + GBS && tid < n, nxt);
-        if((tid >> 5) == blk)
+        const bool owner = (tid >> 5) == blk;
+        if(owner)
         {
-            const int base = r0 & 63; // first lane of the owning half wave
-#pragma unroll
-            for(int k = 0; k < GBS; ++k)
-            {
-                if(r0 + k < n) // uniform
-                {
-                    if(TRANS && lane32 == k)
-#pragma unroll
-                        for(int q = 0; q < NR; ++q)
-                            x[q] = x[q] / cur[k];
-#pragma unroll
-                    for(int q = 0; q < NR; ++q)
-                    {
-                        const TL xk = readlane_t(x[q], base + k);
-                        if(lane32 > k)
-                            x[q] = fma_t(-cur[k], xk, x[q]);
-                    }
-                }
-            }
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 sx[par][lane32][q] = x[q];
+#pragma unroll
+            for(int k = 0; k < GBS; ++k)
+                sL[lane32][k] = cur[k];
         }
         __syncthreads();
-        if(tid >= r0 + GBS && tid < n)
+        solve_diag_block<TL, NR, true, TRANS>(r0, n, sx[par], sL);
+        __syncthreads();
+        if(owner)
+        {
+#pragma unroll
+            for(int q = 0; q < NR; ++q)
+                x[q] = sx[par][lane32][q];
+        }
+        else if(tid >= r0 + GBS && tid < n)
         {
 #pragma unroll
             for(int k = 0; k < GBS; ++k)
@@ -108,7 +164,7 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
 // (U), unit diagonal for op T (L^T).
 template <typename TL, int NR, bool TRANS>
 __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
-                                                 TL (*sx)[GBS][NR])
+                                                 TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
     const int tid = threadIdx.x;
     const int lane32 = tid & (GBS - 1);
@@ -124,33 +180,26 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
         const int par = blk & 1;
         if(blk > 0) // rows above this block meet the previous block column
             load_segment<TL, TRANS>(A, tid, r0 - GBS, n, tid < r0, nxt);
-        if((tid >> 5) == blk)
+        const bool owner = (tid >> 5) == blk;
+        if(owner)
         {
-            const int base = r0 & 63;
-#pragma unroll
-            for(int k = GBS - 1; k >= 0; --k)
-            {
-                if(r0 + k < n) // uniform
-                {
-                    if(!TRANS && lane32 == k)
-#pragma unroll
-                        for(int q = 0; q < NR; ++q)
-                            x[q] = x[q] / cur[k];
-#pragma unroll
-                    for(int q = 0; q < NR; ++q)
-                    {
-                        const TL xk = readlane_t(x[q], base + k);
-                        if(lane32 < k)
-                            x[q] = fma_t(-cur[k], xk, x[q]);
-                    }
-                }
-            }
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 sx[par][lane32][q] = x[q];
+#pragma unroll
+            for(int k = 0; k < GBS; ++k)
+                sL[lane32][k] = cur[k];
         }
         __syncthreads();
-        if(tid < r0)
+        solve_diag_block<TL, NR, false, !TRANS>(r0, n, sx[par], sL);
+        __syncthreads();
+        if(owner)
+        {
+#pragma unroll
+            for(int q = 0; q < NR; ++q)
+                x[q] = sx[par][lane32][q];
+        }
+        else if(tid < r0)
         {
 #pragma unroll
             for(int k = GBS - 1; k >= 0; --k)

@@ -83,7 +83,9 @@ __device__ __forceinline__ int vb_classify(int op, int b, const int* __restrict_
     if(op != VB_GETRF)
     {
         nr = nrhs[b];
-        ok = ok && nr >= 0 && ldb[b] >= (nb > 1 ? nb : 1);
+        const int lb = ldb[b];
+        ok = ok && nr >= 0 && lb >= (nb > 1 ? nb : 1)
+             && uint64_t(lb) * uint64_t(nr) * uint64_t(elem_bytes) < (uint64_t(1) << 31);
     }
     if(!ok)
         return -2;
