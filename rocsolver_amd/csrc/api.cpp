@@ -51,7 +51,7 @@ rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool 
         return reset_info(stream, info, bc);
     if(m <= SMALL_MAX && n <= SMALL_MAX)
         return getrf_small<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
-    if(pivot && m == n && n <= FUSED_MAX)
+    if(pivot && m == n && n <= FUSED_MAX && fits_buffer<T>(lda, n))
         return lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
     return getrf_blocked<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
 }
@@ -233,7 +233,7 @@ rocblas_status dsgetrf_impl(rocblas_handle handle, int n, const double* A, int l
         return reset_info(stream, info, bc);
     Batch<double> Ab = strided(const_cast<double*>(A), strideA);
     Batch<float> Lb = strided(LU, strideLU);
-    if(n > SMALL_MAX && n <= FUSED_MAX)
+    if(n > SMALL_MAX && n <= FUSED_MAX && fits_buffer<double>(lda, n) && fits_buffer<float>(ldlu, n))
         return lu_fused<double, float>(stream, n, Ab, lda, Lb, ldlu, ipiv, strideP, info, bc);
     // other orders: round to FP32 into LU, then the FP32 factorisation in place
     for(int64_t b0 = 0; b0 < bc; b0 += 65535)

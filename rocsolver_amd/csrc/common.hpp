@@ -5,6 +5,8 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 namespace rsa
 {
@@ -68,7 +70,30 @@ struct MatrixBuffer
         else
             return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc, voff, soff, 0));
     }
+    // a store at an out-of-range offset is dropped by the hardware
+    __device__ __forceinline__ void store(T v, uint32_t voff, uint32_t soff) const
+    {
+        if constexpr(sizeof(T) == 8)
+        {
+            using bits_t = decltype(__builtin_amdgcn_raw_buffer_load_b64(rsrc, 0, 0, 0));
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bits_t, v), rsrc, voff, soff, 0);
+        }
+        else
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rsrc, voff, soff, 0);
+    }
+    // the byte offset (row i, column j) for an active lane, else out of range
+    __device__ __forceinline__ uint32_t offset(bool active, int i, int j) const
+    {
+        return active ? row_offset(i) + col_offset(j) : OUT_OF_RANGE;
+    }
 };
+
+// Stops the optimiser from precomputing a sequence of running offsets (it
+// would keep every element of the sequence live in its own register).
+__device__ __forceinline__ void opaque(uint32_t& v)
+{
+    asm volatile("" : "+v"(v));
+}
 
 template <typename T>
 inline bool fits_buffer(int ld, int ncols)
@@ -99,6 +124,19 @@ __device__ __forceinline__ double readlane_t(double v, int l)
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
     return __hiloint2double(hi, lo);
+}
+
+// f(std::integral_constant<int, K>) for K = 0 .. N-1, expanded at compile time
+// (register arrays indexed by K stay in registers whatever the unroller does)
+template <typename F, int... K>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, K...>)
+{
+    (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // Orders LDS accesses between the lanes of one wavefront.  A wavefront's

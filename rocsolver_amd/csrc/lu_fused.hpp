@@ -79,6 +79,10 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     const int lane = tid & 63;
     const int w = tid >> 6;
     const int S = (n + FNB - 1) / FNB;
+    // input and factors through buffer resources (branch-free loads, entries
+    // past the matrix read 0, stores there are dropped)
+    const MatrixBuffer<TI> Ain(Ai, ldi, n);
+    const MatrixBuffer<TC> Aout(Ao, ldo, n);
 
     spi[tid] = tid;
     int zero_at = 0;
@@ -95,17 +99,20 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 #pragma unroll
         for(int kc = 0; kc < NKC; ++kc)
         {
-#pragma unroll
-            for(int j = 0; j < KC; ++j)
             {
-                const int col = c0 + KC * kc + j;
-                TC v = TC(0);
-                if(tid < n && col < n)
+                uint32_t off = Ain.offset(tid < n, tid, c0 + KC * kc);
+                const uint32_t step = tid < n ? Ain.ld_bytes : 0u;
+#pragma unroll
+                for(int j = 0; j < KC; ++j)
                 {
-                    const TI a = Ai[tid + int64_t(col) * ldi];
-                    v = rs ? TC(double(a) * rs[tid] * cs[col]) : TC(a);
+                    const int col = c0 + KC * kc + j;
+                    const TI a = Ain.load(off, 0); // 0 past row or column n
+                    off += step;
+                    opaque(off);
+                    sbuf[j * NMAX + tid]
+                        = rs ? TC(double(a) * rs[tid < n ? tid : 0] * cs[col < n ? col : 0])
+                             : TC(a);
                 }
-                sbuf[j * NMAX + tid] = v;
             }
             __syncthreads();
 #pragma unroll
@@ -132,15 +139,22 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         TC preL[LTT];
         auto issue = [&](int t, int kc) {
             const int r0 = t * FNB, r1 = r0 + FNB;
+            const bool act = tid >= r1 && tid < n;
+            uint32_t off = Aout.offset(act, tid, r0 + KC * kc);
+            const uint32_t step = act ? Aout.ld_bytes : 0u;
 #pragma unroll
             for(int j = 0; j < KC; ++j)
-                pre[j] = (tid >= r1 && tid < n) ? Ao[tid + int64_t(r0 + KC * kc + j) * ldo] : TC(0);
+            {
+                pre[j] = Aout.load(off, 0);
+                off += step;
+                opaque(off);
+            }
             if(kc == 0)
 #pragma unroll
                 for(int e = 0; e < LTT; ++e)
                 {
                     const int idx = tid + e * NMAX, i = idx % FNB, k = idx / FNB;
-                    preL[e] = Ao[(r0 + i) + int64_t(r0 + k) * ldo]; // L_tt (final)
+                    preL[e] = Aout.load(Aout.offset(true, r0 + i, r0 + k), 0); // L_tt (final)
                 }
         };
         if(s > 0)
@@ -387,10 +401,17 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         }
 
         // ---- 4. write block column s in the logical order of this moment
+        {
+            uint32_t off = Aout.offset(label < n, label, c0);
+            const uint32_t step = label < n ? Aout.ld_bytes : 0u;
 #pragma unroll
-        for(int j = 0; j < FNB; ++j)
-            if(j < jb && label < n)
-                Ao[label + int64_t(c0 + j) * ldo] = rowv[j];
+            for(int j = 0; j < FNB; ++j)
+            {
+                Aout.store(rowv[j], off, 0); // columns past n are dropped
+                off += step;
+                opaque(off);
+            }
+        }
         ssig[s * NMAX + tid] = short(tid);
         __syncthreads();
     }
@@ -422,12 +443,12 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     return zero_at;
 }
 
-// At least 2 waves per SIMD: without the bound the compiler spreads into
-// 300-400 registers (1 wave per SIMD); with it the kernel fits 190-240 VGPRs
-// without scratch (a bound of 3 spills), so 2 workgroups of 256 threads
-// share a CU.
+// Waves per SIMD the register budget is cut for: without a bound the
+// compiler spreads into 300-400 registers (1 wave per SIMD).  FP64 fits 2
+// waves (170-240 VGPRs); the FP32 factorisation with 256 or 512 threads fits
+// 4 (128 VGPRs, no scratch), so four 256-thread workgroups share a CU.
 template <typename TI, typename TC, int NMAX, int OPK>
-__global__ __launch_bounds__(NMAX, 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
+__global__ __launch_bounds__(NMAX, (std::is_same<TC, float>::value && OPK == MFMA_NATIVE && NMAX >= 256) ? 4 : 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                                         int ldo, int* __restrict__ ipivA,
                                                         int64_t strideP, int* __restrict__ infoA,
                                                         int bc, const int* __restrict__ skip,

@@ -63,7 +63,7 @@ __device__ __forceinline__ void load_segment(const MatrixBuffer<TL>& A, int i, i
         {
             v[k] = A.load(off, 0);
             off += step;
-            asm volatile("" : "+v"(off));
+            opaque(off);
         }
     }
 }
