@@ -126,19 +126,6 @@ __device__ __forceinline__ double readlane_t(double v, int l)
     return __hiloint2double(hi, lo);
 }
 
-// f(std::integral_constant<int, K>) for K = 0 .. N-1, expanded at compile time
-// (register arrays indexed by K stay in registers whatever the unroller does)
-template <typename F, int... K>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, K...>)
-{
-    (f(std::integral_constant<int, K>{}), ...);
-}
-template <int N, typename F>
-__device__ __forceinline__ void static_for(F&& f)
-{
-    static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // Orders LDS accesses between the lanes of one wavefront.  A wavefront's
 // LDS instructions execute in issue order, so this only has to stop the
 // compiler from moving memory operations across it (no s_barrier: teams of
