@@ -88,6 +88,16 @@ struct MatrixBuffer
     }
 };
 
+// A value equal in every lane, moved to scalar registers (a uniform value
+// the compiler cannot prove uniform would otherwise occupy vector registers
+// for as long as it lives).
+__device__ __forceinline__ double uniform_value(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
 // Stops the optimiser from precomputing a sequence of running offsets (it
 // would keep every element of the sequence live in its own register).
 __device__ __forceinline__ void opaque(uint32_t& v)
@@ -290,6 +300,40 @@ template <typename T>
 __device__ __forceinline__ void wave_argmax(T& key, int& idx)
 {
     group_argmax<64>(key, idx);
+}
+
+// Column maxima over the 64 lanes of a wavefront of K values per lane (K a
+// power of two, 2 <= K <= 64), by halving exchanges: at each step a lane
+// keeps half of its columns and hands the other half to the partner lane
+// (lane ^ 32, then ^ 16, ...), so the steps take K/2 + K/4 + ... + 1
+// exchanges instead of K full reductions; the 64/K lanes that then share a
+// column reduce it.  Every lane returns the maximum (fmax: NaN ignored) of
+// column lane / (64 / K).
+template <int K>
+__device__ __forceinline__ double wave_colmax(const double (&v)[K])
+{
+    static_assert(K >= 2 && K <= 64 && (K & (K - 1)) == 0, "K: power of two in [2, 64]");
+    const int lane = threadIdx.x & 63;
+    double w[K];
+#pragma unroll
+    for(int j = 0; j < K; ++j)
+        w[j] = v[j];
+#pragma unroll
+    for(int half = K / 2, bit = 32; half >= 1; half /= 2, bit /= 2)
+    {
+        const bool up = (lane & bit) != 0; // keeps the upper half of its columns
+#pragma unroll
+        for(int j = 0; j < half; ++j)
+        {
+            const double keep = up ? w[half + j] : w[j];
+            const double send = up ? w[j] : w[half + j];
+            w[j] = fmax(keep, __shfl_xor(send, bit));
+        }
+    }
+#pragma unroll
+    for(int b = 32 / K; b >= 1; b /= 2)
+        w[0] = fmax(w[0], __shfl_xor(w[0], b));
+    return w[0];
 }
 
 inline int ceil_div(int64_t a, int64_t b)

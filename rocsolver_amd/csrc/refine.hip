@@ -63,7 +63,7 @@ __device__ __forceinline__ double pow2_scale(double m)
 // Whole refinement of one system per workgroup (n <= NMAX threads, thread i
 // owns row i).  fskip[b] = 1 when the system needs no FP64 fallback.
 template <int NMAX, int NR>
-__global__ __launch_bounds__(NMAX) void ir_fused_kernel(
+__global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
     int n, int nrhs, Batch<double> AA, int lda, Batch<float> LL, int ldl,
     const int* __restrict__ ipivA, int64_t strideP, const int* __restrict__ info32,
     Batch<double> BB, int ldb, Batch<double> XX, int ldx, int itermax, int* __restrict__ iter,
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     constexpr int NW = NMAX / 64;
     // columns of A per load group in the residual (fewer at 512 rows x 8
     // columns, where the registers are short)
-    constexpr int JU = (NMAX >= 512 && NR >= 8) ? 8 : 16;
+    constexpr int JU = 8;
     __shared__ double sX[NMAX][NR];
     __shared__ float sR[NMAX][NR];
     __shared__ float sx[2][GBS][NR];
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     const MatrixBuffer<double> Ab(A, lda, n);
     const MatrixBuffer<float> LU(LL[b], ldl, n);
     const double* Bm = BB[b];
-    double* X = XX[b];
+    const MatrixBuffer<double> Xbuf(XX[b], ldx, nrhs);
     const bool row = tid < n;
     const MatrixBuffer<double> Bbuf(Bm, ldb, nrhs);
     // B[tid, c] (0 for rows past n or columns past nrhs), branch-free
@@ -173,7 +173,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
     const int pos = pivot_position(n, ipivA + b * strideP, false, sipiv, ssrc);
     (void)pos;
     const int src = ssrc[tid];
-    const double cte_scale = 0x1p-53 * sqrt(double(n));
+    // uniform: kept in scalar registers
+    const double cte_scale = uniform_value(0x1p-53 * sqrt(double(n)));
     double anorm = 0;
     int it_all = 0;
     double worst = 0;
@@ -183,7 +184,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
         const bool measure_a = (c0 == 0);
         // B is re-read at every residual instead of held in registers
         auto bval = [&](int q) { return bload(c0 + q); };
-        double xv[NR];
+        // the FP64 solution lives in LDS only (sX[row][q]); each thread updates its
+        // own row, every thread reads all rows in the residual
         float d[NR];
 
         // correction solve of the FP32 right-hand side rv (row order) -> d
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             const int kj = cexp() + E;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] = ldexp(double(d[q]), kj);
+                sX[tid][q] = row ? ldexp(double(d[q]), kj) : 0.0;
         }
 
         int code = 0; // 0: running, 1: converged, <0: fallback code
@@ -236,10 +238,7 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
         for(;; ++it)
         {
             // ---- R = B - A X (fma over the columns of A in order)
-#pragma unroll
-            for(int q = 0; q < NR; ++q)
-                sX[tid][q] = row ? xv[q] : 0.0;
-            __syncthreads();
+            __syncthreads(); // every row of sX written
             double acc[NR];
 #pragma unroll
             for(int q = 0; q < NR; ++q)
@@ -282,24 +281,29 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
                         cur[u] = nxt[u];
                 }
             }
-            // ---- column maxima of |R| and |X| (and ||A||_inf, max |a| once),
-            //      the largest |R| entry (FP32 range check) and the largest
-            //      exponent of R_scaled r (normalisation of the next solve)
-            double rn[NR], xn[NR], rsmax = 0, remax = NO_EXP;
-            const int ki = rexp();
-#pragma unroll
-            for(int q = 0; q < NR; ++q)
+            // ---- column maxima of |R| and |X| (and ||A||_inf, max |a| once)
+            //      and the largest exponent of R_scaled r (normalisation of
+            //      the next solve).  Per wavefront: one exchange-reduction of
+            //      the 2 NR columns (lane l ends with column l / (32 / NR));
+            //      across wavefronts: lane q combines column q through LDS.
+            constexpr int K = 2 * NR;
+            constexpr int LPC = 64 / K; // lanes per column after the exchanges
+            double remax = NO_EXP;
             {
-                rn[q] = group_max<64>(row ? fabs(acc[q]) : 0.0);
-                xn[q] = group_max<64>(row ? fabs(xv[q]) : 0.0);
-                if(row && c0 + q < nrhs)
+                const int ki = rexp();
+                double v[K];
+#pragma unroll
+                for(int q = 0; q < NR; ++q)
                 {
-                    rsmax = fmax(rsmax, fabs(acc[q]));
-                    if(acc[q] != 0.0 && isfinite(acc[q]))
+                    v[q] = row ? fabs(acc[q]) : 0.0;
+                    v[NR + q] = fabs(sX[tid][q]);
+                    if(row && c0 + q < nrhs && acc[q] != 0.0 && isfinite(acc[q]))
                         remax = fmax(remax, double(ilogb(acc[q]) + ki));
                 }
+                const double cm = wave_colmax<K>(v);
+                if(lane % LPC == 0)
+                    sred[w][lane / LPC] = cm;
             }
-            rsmax = group_max<64>(rsmax);
             remax = group_max<64>(remax);
             if(measure)
             {
@@ -308,43 +312,29 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             }
             if(lane == 0)
             {
-#pragma unroll
-                for(int q = 0; q < NR; ++q)
-                {
-                    sred[w][q] = rn[q];
-                    sred[w][NR + q] = xn[q];
-                }
-                sred[w][2 * NR] = rs;
-                sred[w][2 * NR + 1] = amax;
-                sred[w][2 * NR + 2] = rsmax;
-                sred[w][2 * NR + 3] = remax;
+                sred[w][K] = remax;
+                sred[w][K + 1] = rs;
+                sred[w][K + 2] = amax;
             }
             __syncthreads();
-            rsmax = sred[0][2 * NR + 2];
-            remax = sred[0][2 * NR + 3];
-            for(int v = 1; v < NW; ++v)
-            {
-                rsmax = fmax(rsmax, sred[v][2 * NR + 2]);
-                remax = fmax(remax, sred[v][2 * NR + 3]);
-            }
-#pragma unroll
-            for(int q = 0; q < NR; ++q)
-            {
-                rn[q] = sred[0][q];
-                xn[q] = sred[0][NR + q];
-                for(int v = 1; v < NW; ++v)
+            // lane q < NR: ||R_q||, ||X_q|| of column q; every lane: the rest
+            double rnl = 0, xnl = 0;
+            if(lane < NR)
+                for(int v = 0; v < NW; ++v)
                 {
-                    rn[q] = fmax(rn[q], sred[v][q]);
-                    xn[q] = fmax(xn[q], sred[v][NR + q]);
+                    rnl = fmax(rnl, sred[v][lane]);
+                    xnl = fmax(xnl, sred[v][NR + lane]);
                 }
-            }
+            remax = sred[0][K];
+            for(int v = 1; v < NW; ++v)
+                remax = fmax(remax, sred[v][K]);
             if(measure)
             {
-                double an = sred[0][2 * NR], am = sred[0][2 * NR + 1];
+                double an = sred[0][K + 1], am = sred[0][K + 2];
                 for(int v = 1; v < NW; ++v)
                 {
-                    an = fmax(an, sred[v][2 * NR]);
-                    am = fmax(am, sred[v][2 * NR + 1]);
+                    an = fmax(an, sred[v][K + 1]);
+                    am = fmax(am, sred[v][K + 2]);
                 }
                 anorm = an;
                 if(!scaled && am > double(FLT_MAX)) // the FP32 copy of A overflowed
@@ -354,19 +344,17 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             if(code != 0)
                 break;
             const double cte = anorm * cte_scale;
-            bool conv = true;
+            const bool qv = lane < NR && c0 + lane < nrhs;
+            // dsgesv.f: IF( RNRM.GT.XNRM*CTE ) not converged
+            const bool conv = __builtin_amdgcn_ballot_w64(qv && rnl > xnl * cte) == 0;
             // dlag2s.f on the next right-hand side (normalised on the scaled paths)
-            const bool rbig = !scaled && beyond_fp32(rsmax);
-            err_c = 0;
-#pragma unroll
-            for(int q = 0; q < NR; ++q)
-                if(c0 + q < nrhs)
-                {
-                    if(rn[q] > xn[q] * cte) // dsgesv.f: IF( RNRM.GT.XNRM*CTE ) not converged
-                        conv = false;
-                    const double dn = anorm * xn[q];
-                    err_c = fmax(err_c, dn > 0 ? rn[q] / dn : (rn[q] > 0 ? double(INFINITY) : 0.0));
-                }
+            const bool rbig = !scaled && __builtin_amdgcn_ballot_w64(qv && beyond_fp32(rnl)) != 0;
+            if(conv)
+            {
+                const double dn = anorm * xnl;
+                err_c = group_max<64>(
+                    qv ? (dn > 0 ? rnl / dn : (rnl > 0 ? double(INFINITY) : 0.0)) : 0.0);
+            }
             if(conv)
             {
                 code = 1;
@@ -393,7 +381,8 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
             const int kj = cexp() + E;
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                xv[q] += ldexp(double(d[q]), kj);
+                if(row)
+                    sX[tid][q] += ldexp(double(d[q]), kj);
         }
         if(code < 0)
         {
@@ -402,11 +391,18 @@ __global__ __launch_bounds__(NMAX) void ir_fused_kernel(
         }
         it_all = it > it_all ? it : it_all;
         worst = fmax(worst, err_c);
-        if(row)
+        {
+            // running offset (one register, out of range for rows past n)
+            uint32_t off = Xbuf.offset(row, tid, c0);
 #pragma unroll
             for(int q = 0; q < NR; ++q)
-                if(c0 + q < nrhs)
-                    X[tid + int64_t(c0 + q) * ldx] = xv[q];
+            {
+                if(c0 + q < nrhs) // uniform
+                    Xbuf.store(sX[tid][q], off, 0);
+                off += Xbuf.ld_bytes;
+                opaque(off);
+            }
+        }
     }
     finish(it_all, true, worst);
 }
