@@ -197,8 +197,8 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 d[q] = row ? sR[src][q] : 0.f;
-            solve_lower_rows<float, NR, false>(n, LU, d, sx, sL);
-            solve_upper_rows<float, NR, false>(n, LU, d, sx, sL);
+            solve_lower_rows<float, NR, false, true>(n, LU, d, sx, sL);
+            solve_upper_rows<float, NR, false, true>(n, LU, d, sx, sL);
         };
 
         {
@@ -438,6 +438,82 @@ __global__ __launch_bounds__(NT) void equilibrate_kernel(int n, Batch<double> AA
     }
 }
 
+// In-place inversion of the 32x32 diagonal blocks of an FP32 LU (the
+// factorisation the refinement solves with): the strictly lower part of
+// block k becomes the inverse of its unit-lower L block, the upper part with
+// the diagonal the inverse of its U block, as the triangular-solve kernels of
+// rocBLAS do with their diagonal blocks.  One wavefront per block, the block
+// staged in LDS: lane j < 32 forms column j of inv(L_kk) by forward
+// substitution, lane 32 + j column j of inv(U_kk) by backward substitution
+// (true division by the pivots), each column kept in LDS as it grows.  Rows
+// and columns past n are treated as the identity and not written.  Systems
+// with a zero FP32 pivot are skipped (refinement does not solve with them).
+__global__ __launch_bounds__(64) void invert_diag_blocks_kernel(int n, Batch<float> LL, int ldl,
+                                                                const int* __restrict__ info32,
+                                                                int64_t items)
+{
+    __shared__ float sB[GBS][GBS + 1];
+    __shared__ float sC[GBS][2 * GBS + 1]; // column j of inv(L) at [.][j], of inv(U) at [.][32 + j]
+    const int nblk = (n + GBS - 1) / GBS;
+    const int lane = threadIdx.x;
+    const int i = lane & (GBS - 1); // row in the load/store passes, column j in the solve
+    const bool upper = lane >= GBS;
+    for(int64_t item = blockIdx.x; item < items; item += gridDim.x)
+    {
+        const int64_t b = item / nblk;
+        const int r0 = int(item % nblk) * GBS;
+        if(info32[b] != 0)
+            continue;
+        const int nb = n - r0 < GBS ? n - r0 : GBS;
+        const MatrixBuffer<float> M(LL[b], ldl, n);
+        wave_sync(); // the previous item's reads of sB are done
+#pragma unroll
+        for(int t = 0; t < GBS / 2; ++t)
+        {
+            const int k = (lane >> 5) + 2 * t;
+            const bool in = i < nb && k < nb;
+            const float v = M.load(M.offset(in, r0 + i, r0 + k), 0);
+            sB[i][k] = in ? v : (i == k ? 1.f : 0.f);
+        }
+        wave_sync();
+        const int j = i;
+        if(!upper)
+        {
+#pragma unroll 1
+            for(int r = 0; r < GBS; ++r)
+            {
+                float s = r == j ? 1.f : 0.f;
+#pragma unroll 4
+                for(int m = 0; m < r; ++m)
+                    s = fmaf(-sB[r][m], sC[m][lane], s);
+                sC[r][lane] = s;
+            }
+        }
+        else
+        {
+#pragma unroll 1
+            for(int r = GBS - 1; r >= 0; --r)
+            {
+                float s = r == j ? 1.f : 0.f;
+#pragma unroll 4
+                for(int m = r + 1; m < GBS; ++m)
+                    s = fmaf(-sB[r][m], sC[m][lane], s);
+                sC[r][lane] = s / sB[r][r];
+            }
+        }
+        wave_sync();
+        // rows below the diagonal from inv(L), the rest from inv(U)
+#pragma unroll
+        for(int t = 0; t < GBS / 2; ++t)
+        {
+            const int k = (lane >> 5) + 2 * t;
+            const bool in = i < nb && k < nb;
+            const float v = i > k ? sC[i][k] : sC[i][GBS + k];
+            M.store(v, M.offset(in, r0 + i, r0 + k), 0);
+        }
+    }
+}
+
 // after the FP64 fallback LU: no solve for systems whose FP64 LU is singular
 __global__ void ir_mark_singular_kernel(const int* __restrict__ info, int* __restrict__ fskip,
                                         int bc)
@@ -518,6 +594,15 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
         RSA_LAUNCH_CHECK();
         RSA_TRY(lu_fused_lowp(stream, factor, n, A, lda, LU32, n, ipiv, n, info32, bc, rs, cs,
                               int64_t(n)));
+    }
+    {
+        // diagonal blocks of the FP32 factors -> their inverses (used by every
+        // triangular solve of the refinement)
+        const int64_t items = int64_t((n + GBS - 1) / GBS) * bc;
+        const int grid = int(items < 65536 * 16 ? items : 65536 * 16);
+        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(grid), dim3(64), 0, stream, n, LU32, n,
+                           info32, items);
+        RSA_LAUNCH_CHECK();
     }
     auto by_nr = [&](auto nmax) {
         constexpr int NMAX = decltype(nmax)::value;

@@ -108,13 +108,45 @@ __device__ __forceinline__ void solve_diag_block(int r0, int n, TL (*sx)[NR],
     }
 }
 
+// The same step when the diagonal blocks of the factors have been replaced
+// by their inverses (invert_diag_blocks: the strictly lower part of a block
+// holds the inverse of its unit-lower L block, the upper part with the
+// diagonal the inverse of its U block): the block's solution is a product,
+// y_i = x_i + sum_{k<i} Linv[i][k] x_k (LOWER) or y_i = sum_{k>=i} Uinv[i][k]
+// x_k, so the 32 terms are independent instead of a 32-step chain.  Half
+// wave h computes right-hand sides q = h, h + H, ...; its lanes have read all
+// of column q before any lane overwrites it (one instruction stream, LDS
+// operations in issue order).  Rows and columns past n hold zeros.
+template <typename TL, int NR, bool LOWER>
+__device__ __forceinline__ void apply_inv_block(TL (*sx)[NR], const TL (*sL)[GBS + 1])
+{
+    const int tid = threadIdx.x;
+    const int i = tid & (GBS - 1);
+    const int h = tid >> 5;
+    const int H = blockDim.x >> 5;
+    for(int q = h; q < NR; q += H)
+    {
+        TL y = LOWER ? sx[i][q] : TL(0);
+#pragma unroll 8
+        for(int k = 0; k < GBS; ++k)
+        {
+            // predicated rather than multiplied by zero: an infinite x_k
+            // outside the triangle must not turn y into NaN
+            const TL t = fma_t(sL[i][k], sx[k][q], y);
+            y = (LOWER ? k < i : k >= i) ? t : y;
+        }
+        sx[i][q] = y;
+    }
+}
+
 // Forward substitution with the lower triangle of op(A): unit diagonal for
 // op N (L), non-unit for op T (U^T).  sx: LDS [2][GBS][NR]; sL: LDS
-// [GBS][GBS+1].
-template <typename TL, int NR, bool TRANS>
+// [GBS][GBS+1].  INV (op N only): the diagonal blocks hold their inverses.
+template <typename TL, int NR, bool TRANS, bool INV = false>
 __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
+    static_assert(!(INV && TRANS), "inverted diagonal blocks: op N only");
     const int tid = threadIdx.x;
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
@@ -137,7 +169,10 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
                 sL[lane32][k] = cur[k];
         }
         __syncthreads();
-        solve_diag_block<TL, NR, true, TRANS>(r0, n, sx[par], sL);
+        if constexpr(INV)
+            apply_inv_block<TL, NR, true>(sx[par], sL);
+        else
+            solve_diag_block<TL, NR, true, TRANS>(r0, n, sx[par], sL);
         __syncthreads();
         if(owner)
         {
@@ -161,11 +196,12 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
 }
 
 // Backward substitution with the upper triangle of op(A): non-unit for op N
-// (U), unit diagonal for op T (L^T).
-template <typename TL, int NR, bool TRANS>
+// (U), unit diagonal for op T (L^T).  INV as for solve_lower_rows.
+template <typename TL, int NR, bool TRANS, bool INV = false>
 __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
+    static_assert(!(INV && TRANS), "inverted diagonal blocks: op N only");
     const int tid = threadIdx.x;
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
@@ -191,7 +227,10 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
                 sL[lane32][k] = cur[k];
         }
         __syncthreads();
-        solve_diag_block<TL, NR, false, !TRANS>(r0, n, sx[par], sL);
+        if constexpr(INV)
+            apply_inv_block<TL, NR, false>(sx[par], sL);
+        else
+            solve_diag_block<TL, NR, false, !TRANS>(r0, n, sx[par], sL);
         __syncthreads();
         if(owner)
         {
