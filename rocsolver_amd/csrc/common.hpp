@@ -98,6 +98,16 @@ __device__ __forceinline__ double uniform_value(double v)
     return __hiloint2double(hi, lo);
 }
 
+// threadIdx.x, opaque to the optimiser: addresses derived from it are then
+// computed where they are used instead of once at kernel entry and kept live
+// (in registers, or spilled) through every phase of a long kernel.
+__device__ __forceinline__ int fresh_tid()
+{
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // Stops the optimiser from precomputing a sequence of running offsets (it
 // would keep every element of the sequence live in its own register).
 __device__ __forceinline__ void opaque(uint32_t& v)

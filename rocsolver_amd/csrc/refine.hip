@@ -237,6 +237,10 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
         double err_c = 0;
         for(;; ++it)
         {
+            // per-lane addresses recomputed in every step, not kept from entry
+            const int tid = fresh_tid();
+            const int lane = tid & 63, w = tid >> 6;
+            const bool row = tid < n;
             // ---- R = B - A X (fma over the columns of A in order)
             __syncthreads(); // every row of sX written
             double acc[NR];

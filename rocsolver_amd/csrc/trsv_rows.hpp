@@ -81,7 +81,7 @@ template <typename TL, int NR, bool LOWER, bool NONUNIT>
 __device__ __forceinline__ void solve_diag_block(int r0, int n, TL (*sx)[NR],
                                                  const TL (*sL)[GBS + 1])
 {
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const int i = tid & (GBS - 1);
     const int h = tid >> 5;
     const int H = blockDim.x >> 5;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void solve_diag_block(int r0, int n, TL (*sx)[NR],
 template <typename TL, int NR, bool LOWER>
 __device__ __forceinline__ void apply_inv_block(TL (*sx)[NR], const TL (*sL)[GBS + 1])
 {
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const int i = tid & (GBS - 1);
     const int h = tid >> 5;
     const int H = blockDim.x >> 5;
@@ -147,7 +147,7 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
     static_assert(!(INV && TRANS), "inverted diagonal blocks: op N only");
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
     TL cur[GBS], nxt[GBS];
@@ -202,7 +202,7 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
     static_assert(!(INV && TRANS), "inverted diagonal blocks: op N only");
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
     TL cur[GBS], nxt[GBS];
@@ -261,7 +261,7 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
 __device__ __forceinline__ int pivot_position(int n, const int* __restrict__ ipiv, bool trans,
                                               int* sipiv, int* ssrc)
 {
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     int p_in = tid < n ? ipiv[tid] - 1 : tid;
     if(p_in < 0 || p_in >= n)
         p_in = tid;
