@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 N, BC, NRHS = 256, 4096, 8
 FP32_MFMA_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+HBM_PEAK_TBS = 8.0  # MI355X HBM3E peak
 EPS64 = 2.0 ** -52
 
 
@@ -276,6 +277,84 @@ def measured_traffic(kernel: str):
     return int(rec["fetch_bytes"] + rec["write_bytes"])
 
 
+GETRF32_N, GETRF32_BATCH = 32, 131072
+
+
+def bench_getrf32(args) -> int:
+    """Plain FP64 rocsolver_dgetrf_strided_batched at n = 32 (lda 32, stride
+    1024, entries U(-1,1)), batch 131072 (2.2 GB of HBM traffic per call, so
+    launch latency does not enter the roofline figure).  A step is one call;
+    the input is restored before each step with a device copy queued ahead of
+    the start event, so the GPU never waits for the host inside the timed
+    region.  Roofline: HBM bytes per matrix = A read + A written + ipiv + info."""
+    import numpy as np
+    import torch
+
+    import rocsolver_amd as ra
+    torch.cuda.set_device(0)
+    n, bc = GETRF32_N, GETRF32_BATCH
+    g = torch.Generator(device="cuda").manual_seed(11)
+    A0 = torch.rand((bc, n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    A = torch.empty_like(A0)
+    ipiv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+    handle = ra.Handle()
+
+    def step():
+        ra.check("rocsolver_dgetrf_strided_batched", handle.value, n, n, A.data_ptr(), n, n * n,
+                 ipiv.data_ptr(), n, info.data_ptr(), bc)
+    s = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        A.copy_(A0)
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(args.steps):
+        A.copy_(A0)
+        e0.record(s)
+        step()
+        e1.record(s)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    t = float(np.mean(times))
+    per = 2 * n * n * 8 + n * 4 + 4
+    gbs = bc * per / t / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle import lapack
+        if lapack.available():
+            lapack.set_num_threads(1)
+            sample = 16384
+            Ah = A0[:sample].cpu().numpy().copy()
+            t0 = time.perf_counter()
+            lapack.getrf(Ah, n)
+            t1 = time.perf_counter()
+            cpu = {"value": round(sample / (t1 - t0), 1), "unit": "matrices/s", "cores": 1,
+                   "kind": "reference",
+                   "sample": (f"dgetrf_ of {sample} matrices of the same batch through host "
+                              f"LAPACK ({lapack.load()[2]}), one thread")}
+    out = {
+        "metric": "n=32 dgetrf_strided_batched HBM roofline fraction",
+        "value": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
+        "unit": "fraction of 8 TB/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded U(-1,1), generated on the GPU)",
+        "config": {"workload": "plain FP64 dgetrf_strided_batched, n=32, lda=32, stride 1024",
+                   "n": n, "batch": bc},
+        "matrices_per_s": round(bc / t, 1),
+        "nonzero_info": int((info != 0).sum()),
+        "roofline": {"kernel": "getrf_small_kernel<double,32,32,pivot> (team of 32 lanes per matrix)",
+                     "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
+                     "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
+                     "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,9 +363,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of single-thread host LAPACK in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["mixed", "vbatched"], default="mixed",
+    ap.add_argument("--workload", choices=["mixed", "vbatched", "getrf32"], default="mixed",
                     help="mixed: the headline n=256 mixed-precision solve (default); vbatched: "
-                         "10000 matrices of orders 8..512 in one persistent launch")
+                         "10000 matrices of orders 8..512 in one persistent launch; getrf32: "
+                         "HBM roofline of plain n=32 dgetrf_strided_batched")
     ap.add_argument("--tune", action="store_true",
                     help="vbatched: run the schedule autotuner on the workload first")
     args = ap.parse_args()
@@ -297,6 +377,8 @@ def main() -> int:
         return 2
     if args.workload == "vbatched":
         return bench_vbatched(args)
+    if args.workload == "getrf32":
+        return bench_getrf32(args)
 
     import torch
 

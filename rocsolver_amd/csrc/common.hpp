@@ -47,8 +47,12 @@ struct MatrixBuffer
     uint32_t ld_bytes;
 
     __device__ __forceinline__ MatrixBuffer(const T* base, int ld, int ncols)
+        : MatrixBuffer(base, ld, uint64_t(ld) * uint64_t(ncols) * sizeof(T))
     {
-        const uint64_t bytes = uint64_t(ld) * uint64_t(ncols) * sizeof(T);
+    }
+    // a buffer of `bytes` bytes from base (several matrices of one wavefront)
+    __device__ __forceinline__ MatrixBuffer(const T* base, int ld, uint64_t bytes)
+    {
         // word 3 of a raw gfx9 buffer resource: 32-bit data format, no swizzle
         rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0,
                                                  int(bytes < OUT_OF_RANGE ? bytes : OUT_OF_RANGE - 1),
@@ -295,15 +299,69 @@ __device__ __forceinline__ int group_min(int v)
     return v;
 }
 
-// Arg-max of (key, index) over a group of TS lanes, ties to the smaller
-// index; every lane of the group ends with the winner.  Keys must not be
-// NaN (pivot_key maps NaN to +-inf / -1).
-template <int TS, typename T>
-__device__ __forceinline__ void group_argmax(T& key, int& idx)
+// Largest signed / unsigned 32-bit integer over the group (every lane gets
+// it); the compiler folds each DPP move into the max (v_max_*_dpp).
+template <int TS, typename I>
+__device__ __forceinline__ I group_max_int(I v)
 {
-    const T m = group_max<TS>(key);
-    idx = group_min<TS>(key == m ? idx : 0x7fffffff);
-    key = m;
+    auto mx = [](I a, I b) { return a > b ? a : b; };
+    v = mx(v, I(dpp_mov<0xB1>(int(v))));
+    v = mx(v, I(dpp_mov<0x4E>(int(v))));
+    v = mx(v, I(dpp_mov<0x141>(int(v))));
+    v = mx(v, I(dpp_mov<0x140>(int(v))));
+    if(TS >= 32)
+    {
+        lane_pair p = swap16(int(v));
+        v = mx(I(p.a), I(p.b));
+    }
+    if(TS >= 64)
+    {
+        lane_pair p = swap32(int(v));
+        v = mx(I(p.a), I(p.b));
+    }
+    return v;
+}
+
+// Arg-max of (key, index) over a group of TS lanes, ties to the smaller
+// index; every lane of the group ends with the winner.  Keys are pivot_key
+// values: non-negative (NaN already mapped to +inf) or -1 for a lane that is
+// not a candidate.  The IEEE bit patterns of non-negative values order like
+// the values, so the maximum is found on integers -- for double the high
+// words first, then the low words among the lanes that hold the largest high
+// word -- with DPP folded into each integer max, where a floating-point max
+// would need a separate DPP move and NaN canonicalisation per step.
+// Non-candidates enter with key bits 0 and lose every tie to a candidate
+// (index tagged with bit 30); with no candidate at all the result is the
+// smallest index and key -1, as a maximum over the -1 keys would give (an
+// index of INT_MAX stands for "no row" and stays INT_MAX).
+// Indices must be below 2^30 (or INT_MAX for "none").
+template <int TS>
+__device__ __forceinline__ void group_argmax(double& key, int& idx)
+{
+    const bool cand = key >= 0.0;
+    const uint64_t u = cand ? uint64_t(__double_as_longlong(key)) : 0u;
+    const int hi = int(u >> 32); // < 2^31: the sign bit is clear
+    const unsigned lo = unsigned(u);
+    const int mh = group_max_int<TS>(hi);
+    const unsigned ml = group_max_int<TS>(hi == mh ? lo : 0u);
+    constexpr int TAG = 0x40000000;
+    const int r = group_min<TS>((hi == mh && lo == ml) ? (cand ? idx : (idx | TAG)) : 0x7fffffff);
+    const bool none = r == 0x7fffffff || (r & TAG) != 0;
+    idx = (r == 0x7fffffff) ? r : (r & ~TAG);
+    key = none ? -1.0 : __longlong_as_double((long long)((uint64_t(unsigned(mh)) << 32) | ml));
+}
+
+template <int TS>
+__device__ __forceinline__ void group_argmax(float& key, int& idx)
+{
+    const bool cand = key >= 0.f;
+    const int u = cand ? __float_as_int(key) : 0; // < 2^31: the sign bit is clear
+    const int m = group_max_int<TS>(u);
+    constexpr int TAG = 0x40000000;
+    const int r = group_min<TS>(u == m ? (cand ? idx : (idx | TAG)) : 0x7fffffff);
+    const bool none = r == 0x7fffffff || (r & TAG) != 0;
+    idx = (r == 0x7fffffff) ? r : (r & ~TAG);
+    key = none ? -1.f : __int_as_float(m);
 }
 
 template <typename T>

@@ -36,6 +36,7 @@ namespace rsa
 // One team factors one matrix held in registers r[NC].  On return `label`
 // is this lane's logical row, `mypiv` its ipiv entry (1-based, valid when
 // label < min(m,n)) and `zero_at` the first zero pivot (1-based) or 0.
+// prow: NC LDS elements for this team (16-byte aligned).
 template <typename T, int NC, int TS, bool PIVOT>
 __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool live,
                                         int& label, int& mypiv, int& zero_at)
@@ -57,9 +58,13 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
             const bool is_piv = (label == plab);
             if(is_piv)
             {
+                // whole 16-byte groups from the one holding column k (an
+                // aligned ds_write_b128 each; entries left of k are unused)
+                constexpr int G = 16 / int(sizeof(T));
 #pragma unroll
-                for(int j = k; j < NC; ++j)
-                    prow[j] = r[j];
+                for(int j = 0; j < NC; ++j)
+                    if(j / G >= k / G)
+                        prow[j] = r[j];
             }
             wave_sync();
             const T pv = prow[k];
@@ -87,18 +92,57 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
     }
 }
 
+// Element (i, j) of a team's column-major matrix, through a plain global
+// pointer (any batch layout) ...
+template <typename T>
+struct PtrAccess
+{
+    T* a;
+    int ld;
+    __device__ __forceinline__ T load(bool live, int i, int j) const
+    {
+        return live ? a[i + int64_t(j) * ld] : T(0);
+    }
+    __device__ __forceinline__ void store(int i, int j, T v) const
+    {
+        a[i + int64_t(j) * ld] = v;
+    }
+};
+
+// ... or through one buffer resource for the whole wavefront (strided
+// batches whose matrices of one wavefront span less than 2 GiB): the team's
+// matrix starts team_off bytes into the buffer, the row offset is one vector
+// register and the column offset a scalar, so a column costs one load or
+// store instruction and no address arithmetic.
+template <typename T>
+struct BufAccess
+{
+    MatrixBuffer<T> buf;
+    uint32_t team_off;
+    __device__ __forceinline__ T load(bool live, int i, int j) const
+    {
+        return buf.load(live ? team_off + uint32_t(i) * uint32_t(sizeof(T))
+                             : MatrixBuffer<T>::OUT_OF_RANGE,
+                        buf.col_offset(j));
+    }
+    __device__ __forceinline__ void store(int i, int j, T v) const
+    {
+        buf.store(v, team_off + uint32_t(i) * uint32_t(sizeof(T)), buf.col_offset(j));
+    }
+};
+
 // getrf of one m x n matrix by the team (row = lane index within the team,
 // valid = the team has a matrix).  prow: LDS, NC elements for this team.
 // Returns the first zero pivot (1-based) or 0, in every lane of the team.
-template <typename T, int NC, int TS, bool PIVOT>
-__device__ __forceinline__ int team_getrf(int row, bool valid, int m, int n, T* Ab, int lda,
+template <typename T, int NC, int TS, bool PIVOT, typename Acc>
+__device__ __forceinline__ int team_getrf(int row, bool valid, int m, int n, const Acc& Am,
                                           int* ipivb, T* prow)
 {
     const bool live = valid && row < m;
     T r[NC];
 #pragma unroll
     for(int j = 0; j < NC; ++j)
-        r[j] = (live && j < n) ? Ab[row + int64_t(j) * lda] : T(0);
+        r[j] = (j < n) ? Am.load(live, row, j) : T(0);
 
     int label = row, mypiv = row + 1, zero_at = 0;
     team_lu<T, NC, TS, PIVOT>(r, prow, m, n, live, label, mypiv, zero_at);
@@ -109,11 +153,18 @@ __device__ __forceinline__ int team_getrf(int row, bool valid, int m, int n, T* 
 #pragma unroll
         for(int j = 0; j < NC; ++j)
             if(j < n)
-                Ab[label + int64_t(j) * lda] = r[j];
+                Am.store(label, j, r[j]);
         if(PIVOT && label < kmax)
             ipivb[label] = mypiv;
     }
     return zero_at;
+}
+
+template <typename T, int NC, int TS, bool PIVOT>
+__device__ __forceinline__ int team_getrf(int row, bool valid, int m, int n, T* Ab, int lda,
+                                          int* ipivb, T* prow)
+{
+    return team_getrf<T, NC, TS, PIVOT>(row, valid, m, n, PtrAccess<T>{Ab, lda}, ipivb, prow);
 }
 
 // Fused gesv of one n x n system by the team: factor in registers, then
@@ -402,21 +453,34 @@ __device__ __forceinline__ void team_getrs(int row, bool valid, bool trans, int 
 // ------------------------------------------------------------- kernels
 // One wavefront per workgroup, 64/TS matrices of the same order per wave.
 
-template <typename T, int NC, int TS, bool PIVOT>
+// BUF: strided batch read and written through one buffer resource per
+// wavefront (the host checks that the wavefront's matrices span < 2 GiB).
+template <typename T, int NC, int TS, bool PIVOT, bool BUF>
 __global__ __launch_bounds__(64) void getrf_small_kernel(int m, int n, Batch<T> A, int lda,
                                                          int* __restrict__ ipiv, int64_t strideP,
                                                          int* __restrict__ info, int bc)
 {
     constexpr int TPW = 64 / TS;
-    __shared__ T prow[TPW][NC];
+    __shared__ __align__(16) T prow[TPW][NC];
     const int lane = threadIdx.x;
     const int team = lane / TS;
     const int row = lane % TS;
-    const int64_t b = int64_t(blockIdx.x) * TPW + team;
+    const int64_t b0 = int64_t(blockIdx.x) * TPW;
+    const int64_t b = b0 + team;
     const bool valid = b < bc;
-    const int zero_at = team_getrf<T, NC, TS, PIVOT>(row, valid, m, n, valid ? A[b] : nullptr,
-                                                     lda, ipiv + (valid ? b : 0) * strideP,
-                                                     prow[team]);
+    int zero_at;
+    if constexpr(BUF)
+    {
+        const uint64_t span
+            = (uint64_t(TPW - 1) * uint64_t(A.stride) + uint64_t(lda) * uint64_t(n)) * sizeof(T);
+        const BufAccess<T> Am{MatrixBuffer<T>(A[b0], lda, span),
+                              uint32_t(team) * uint32_t(A.stride) * uint32_t(sizeof(T))};
+        zero_at = team_getrf<T, NC, TS, PIVOT>(row, valid, m, n, Am,
+                                               ipiv + (valid ? b : 0) * strideP, prow[team]);
+    }
+    else
+        zero_at = team_getrf<T, NC, TS, PIVOT>(row, valid, m, n, valid ? A[b] : nullptr, lda,
+                                               ipiv + (valid ? b : 0) * strideP, prow[team]);
     if(valid && row == 0)
         info[b] = zero_at;
 }
@@ -428,7 +492,7 @@ __global__ __launch_bounds__(64) void gesv_small_kernel(int n, int nrhs, Batch<T
                                                         int* __restrict__ info, int bc)
 {
     constexpr int TPW = 64 / TS;
-    __shared__ T prow[TPW][NC];
+    __shared__ __align__(16) T prow[TPW][NC];
     __shared__ T pb[TPW][NR];
     const int lane = threadIdx.x;
     const int team = lane / TS;
@@ -531,12 +595,24 @@ rocblas_status getrf_small_launch(hipStream_t stream, int m, int n, Batch<T> A, 
             constexpr int TS = decltype(ts)::value;
             constexpr int NC = decltype(nc)::value;
             dim3 grid(ceil_div(bc, 64 / TS));
-            if(pivot)
-                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, true>), grid, dim3(64), 0,
+            // one buffer per wavefront when its matrices span < 2 GiB
+            const bool buf = A.ptrs == nullptr && A.stride >= 0
+                             && (uint64_t(64 / TS - 1) * uint64_t(A.stride)
+                                 + uint64_t(lda) * uint64_t(n))
+                                        * sizeof(T)
+                                    < (uint64_t(1) << 31);
+            if(pivot && buf)
+                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, true, true>), grid, dim3(64), 0,
                                    stream, m, n, A, lda, ipiv, strideP, info, bc);
+            else if(pivot)
+                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, true, false>), grid, dim3(64),
+                                   0, stream, m, n, A, lda, ipiv, strideP, info, bc);
+            else if(buf)
+                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, false, true>), grid, dim3(64),
+                                   0, stream, m, n, A, lda, ipiv, strideP, info, bc);
             else
-                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, false>), grid, dim3(64), 0,
-                                   stream, m, n, A, lda, ipiv, strideP, info, bc);
+                hipLaunchKernelGGL((getrf_small_kernel<T, NC, TS, false, false>), grid, dim3(64),
+                                   0, stream, m, n, A, lda, ipiv, strideP, info, bc);
         });
     });
     RSA_LAUNCH_CHECK();

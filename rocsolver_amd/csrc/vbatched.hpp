@@ -168,7 +168,7 @@ __global__ __launch_bounds__(VB_THREADS, 2) void vb_work_kernel(VbArgs<T> a, VbT
                                                                 int* __restrict__ ctrl)
 {
     __shared__ int s_item;
-    __shared__ T prow[VB_WAVES][4][64];
+    __shared__ __align__(16) T prow[VB_WAVES][4][64];
     __shared__ T pb[VB_WAVES][4][VB_NR];
     __shared__ int perm[OP == VB_GETRS ? VB_WAVES : 1][4][64];
     __shared__ int spiv[OP == VB_GETRS ? VB_WAVES : 1][4][64];
