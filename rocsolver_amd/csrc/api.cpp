@@ -282,15 +282,15 @@ rocblas_status vbatched_impl(rocblas_handle handle, int op, rocblas_operation tr
 {
     if(!handle)
         return rocblas_status_invalid_handle;
-    // getrf / getrs alone are switched off: their first GPU test run ended in a
-    // failure followed by a hang whose cause has not been found; gesv is on
-    if(op != 2)
+    // getrf alone is switched off: its first GPU run was followed by a hang
+    // whose cause has not been found (getrs and gesv are on)
+    if(op == 0)
         return rocblas_status_not_implemented;
     bool tr = false;
     if(op == 1)
     {
         rocblas_status st = check_trans(trans, &tr);
-        if(st != rocblas_status_success)
+        if(st != rocblas_status_continue)
             return st;
     }
     if(bc < 0)

@@ -101,9 +101,13 @@ def test_vbatched_argument_checks(p):
                    1) == 1
     assert ra.call(f"rocsolver_amd_{p}gesv_vbatched", None, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
                    DUMMY, DUMMY, DUMMY, 1) == 1
-    # getrf / getrs alone are switched off for now (rocblas_status_not_implemented = 2)
+    # getrs: an invalid trans is reported before the sizes; a valid one passes
+    # on to the size check (rocblas_operation_none = 111)
     assert ra.call(f"rocsolver_amd_{p}getrs_vbatched", DUMMY, 999, DUMMY, DUMMY, DUMMY, DUMMY,
-                   DUMMY, DUMMY, DUMMY, None, -1) == 2
+                   DUMMY, DUMMY, DUMMY, None, -1) == 11
+    assert ra.call(f"rocsolver_amd_{p}getrs_vbatched", DUMMY, 111, DUMMY, DUMMY, DUMMY, DUMMY,
+                   DUMMY, DUMMY, DUMMY, None, -1) == 4
+    # getrf alone is switched off for now (rocblas_status_not_implemented = 2)
     assert ra.call(f"rocsolver_amd_{p}getrf_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
                    -1) == 2
     assert ra.call(f"rocsolver_amd_{p}gesv_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,

@@ -55,7 +55,8 @@ def test_getrf_reference_matrices(torch_gpu, handle, m, lda, singular, n, dtype,
     np.testing.assert_array_equal(Ag[:, :, m:], A0[:, :, m:])
 
 
-@pytest.mark.parametrize("m,n", [(1, 1), (3, 7), (7, 3), (16, 16), (33, 33), (64, 64), (64, 8),
+@pytest.mark.parametrize("m,n", [(1, 1), (3, 7), (7, 3), (16, 16), (17, 17), (20, 31), (31, 20),
+                                 (24, 32), (32, 9), (32, 32), (33, 33), (64, 64), (64, 8),
                                  (8, 64), (65, 65), (96, 96), (128, 128), (130, 70), (70, 130),
                                  (256, 256), (300, 300), (512, 512), (600, 600)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -82,6 +83,35 @@ def test_getrf_random(torch_gpu, handle, m, n, dtype):
     assert be.max() < 10.0, be
     for b in np.nonzero(unamb)[0]:
         assert rel_fro(Ag[b], Ao[b]) <= 10 * min(m, n) * EPS[dtype]
+
+
+@pytest.mark.parametrize("m", [20, 32])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_getrf_small_special_values(torch_gpu, handle, m, dtype):
+    """Register-resident LU on matrices with NaN, +-inf, a zero column and
+    exactly tied pivot candidates, batch 7 (a partial last wavefront):
+    ipiv, info and the factors equal the oracle's unblocked LU bit for bit
+    (NaN positions included)."""
+    print(f"getrf special values m={m} {dtype.__name__}", flush=True)
+    g = np.random.default_rng(m + 3)
+    bc = 7
+    A0 = g.uniform(-1, 1, size=(bc, m, m)).astype(dtype)  # [b, col, row]
+    A0[1, 3, 5] = np.nan               # NaN below the diagonal of column 3
+    A0[2, 0, 0] = np.nan               # NaN on the first pivot candidate
+    A0[3, 2, 7] = np.inf
+    A0[3, 4, 1] = -np.inf
+    A0[4, 6, :] = 0.0                  # zero column -> info = 7
+    A0[5, 0, :] = 0.5                  # column 0 all equal: ties to the lowest row
+    A0[5, 1, ::2] = -0.25
+    A0[6] = np.round(A0[6] * 4) / 4    # many exact ties throughout
+    Ao = A0.copy()
+    ip_o, in_o = oracle.getf2(Ao, m)
+    Ad = dev(torch_gpu, A0)
+    ip, info = ra.getrf(handle, Ad, m=m)
+    torch_gpu.cuda.synchronize()
+    np.testing.assert_array_equal(host(info), in_o)
+    np.testing.assert_array_equal(host(ip), ip_o)
+    np.testing.assert_array_equal(host(Ad), Ao)
 
 
 @pytest.mark.parametrize("n", [4, 32, 100])
