@@ -446,73 +446,88 @@ __global__ __launch_bounds__(NT) void equilibrate_kernel(int n, Batch<double> AA
 // factorisation the refinement solves with): the strictly lower part of
 // block k becomes the inverse of its unit-lower L block, the upper part with
 // the diagonal the inverse of its U block, as the triangular-solve kernels of
-// rocBLAS do with their diagonal blocks.  One wavefront per block, the block
-// staged in LDS: lane j < 32 forms column j of inv(L_kk) by forward
-// substitution, lane 32 + j column j of inv(U_kk) by backward substitution
-// (true division by the pivots), each column kept in LDS as it grows.  Rows
-// and columns past n are treated as the identity and not written.  Systems
-// with a zero FP32 pivot are skipped (refinement does not solve with them).
-__global__ __launch_bounds__(64) void invert_diag_blocks_kernel(int n, Batch<float> LL, int ldl,
-                                                                const int* __restrict__ info32,
-                                                                int64_t items)
+// rocBLAS do with their diagonal blocks.  One wavefront per block: lane j < 32
+// forms column j of inv(L_kk), lane 32 + j column j of inv(U~), where U~ =
+// J U_kk J (J reverses the index order) is lower triangular, so both halves
+// run the same column-oriented forward substitution -- x_m /= d_m, then
+// x_i -= a_im x_m for every i > m, 31 - m independent fmas per step -- on
+// their own matrix staged in LDS (d_m = 1 for L: an exact division).  The
+// column of inv(U) is the reversed column of inv(U~).  Rows and columns past
+// n are treated as the identity and not written.  Systems with a zero FP32
+// pivot are skipped (refinement does not solve with them).
+__global__ __launch_bounds__(64, 5) void invert_diag_blocks_kernel(int n, Batch<float> LL, int ldl,
+                                                                const int* __restrict__ info32)
 {
-    __shared__ float sB[GBS][GBS + 1];
-    __shared__ float sC[GBS][2 * GBS + 1]; // column j of inv(L) at [.][j], of inv(U) at [.][32 + j]
+    // sM[h][m][i]: column m of the half's lower-triangular matrix (h = 0: L,
+    // h = 1: U~), so a step reads its column as 16-byte broadcasts; once the
+    // solve has read it, the same LDS holds the results sC[h][row][col] of
+    // inv(L) and inv(U~) (one buffer: a one-wave workgroup's LDS sets the
+    // occupancy)
+    __shared__ __align__(16) float sbuf[2 * GBS * (GBS + 1)];
+    float(*sM)[GBS][GBS] = reinterpret_cast<float(*)[GBS][GBS]>(sbuf);
+    float(*sC)[GBS][GBS + 1] = reinterpret_cast<float(*)[GBS][GBS + 1]>(sbuf);
     const int nblk = (n + GBS - 1) / GBS;
     const int lane = threadIdx.x;
     const int i = lane & (GBS - 1); // row in the load/store passes, column j in the solve
-    const bool upper = lane >= GBS;
-    for(int64_t item = blockIdx.x; item < items; item += gridDim.x)
+    const int h = lane >> 5;
     {
-        const int64_t b = item / nblk;
-        const int r0 = int(item % nblk) * GBS;
+        // one block per workgroup (no loop over items: the per-lane LDS
+        // addresses of the unrolled solve would be kept live across it)
+        const int64_t b = blockIdx.x / nblk;
+        const int r0 = int(blockIdx.x % nblk) * GBS;
         if(info32[b] != 0)
-            continue;
+            return;
         const int nb = n - r0 < GBS ? n - r0 : GBS;
         const MatrixBuffer<float> M(LL[b], ldl, n);
-        wave_sync(); // the previous item's reads of sB are done
 #pragma unroll
         for(int t = 0; t < GBS / 2; ++t)
         {
             const int k = (lane >> 5) + 2 * t;
             const bool in = i < nb && k < nb;
             const float v = M.load(M.offset(in, r0 + i, r0 + k), 0);
-            sB[i][k] = in ? v : (i == k ? 1.f : 0.f);
+            const float e = in ? v : (i == k ? 1.f : 0.f); // block entry (i, k)
+            if(i > k)
+                sM[0][k][i] = e; // L, below the diagonal
+            else
+                sM[1][GBS - 1 - k][GBS - 1 - i] = e; // U~(31-i, 31-k) = U(i, k), i <= k
         }
         wave_sync();
         const int j = i;
-        if(!upper)
+        float x[GBS];
+#pragma unroll
+        for(int r = 0; r < GBS; ++r)
+            x[r] = r == j ? 1.f : 0.f;
+        uint32_t base = uint32_t(h) * uint32_t(GBS * GBS); // this half's matrix in sM
+#pragma unroll
+        for(int m = 0; m < GBS; ++m)
         {
-#pragma unroll 1
-            for(int r = 0; r < GBS; ++r)
-            {
-                float s = r == j ? 1.f : 0.f;
-#pragma unroll 4
-                for(int m = 0; m < r; ++m)
-                    s = fmaf(-sB[r][m], sC[m][lane], s);
-                sC[r][lane] = s;
-            }
+            // the column's reads wait for x[m] (an empty asm ties the address
+            // to it): otherwise every read of the unrolled solve is hoisted to
+            // the top and the 496 values do not fit in registers
+            asm volatile("" : "+v"(base) : "v"(x[m]));
+            // column m of this half's matrix, rows m..31 (row m: the diagonal)
+            const float* col = sbuf + base + m * GBS;
+            const float dv = col[m]; // read by both halves (select, no branch)
+            x[m] = x[m] / (h ? dv : 1.f);
+#pragma unroll
+            for(int r = m + 1; r < GBS; ++r)
+                x[r] = fmaf(-col[r], x[m], x[r]);
         }
-        else
-        {
-#pragma unroll 1
-            for(int r = GBS - 1; r >= 0; --r)
-            {
-                float s = r == j ? 1.f : 0.f;
-#pragma unroll 4
-                for(int m = r + 1; m < GBS; ++m)
-                    s = fmaf(-sB[r][m], sC[m][lane], s);
-                sC[r][lane] = s / sB[r][r];
-            }
-        }
+        wave_sync(); // every lane has read sM: it becomes sC
+        // the whole column into the half's own region (no per-row condition)
+        float* dst = &sC[h][0][j];
+#pragma unroll
+        for(int r = 0; r < GBS; ++r)
+            dst[r * (GBS + 1)] = x[r];
         wave_sync();
-        // rows below the diagonal from inv(L), the rest from inv(U)
+        // entry (i, k): inv(L)[i][k] below the diagonal; inv(U)[i][k] =
+        // inv(U~)[31-i][31-k] on and above it
 #pragma unroll
         for(int t = 0; t < GBS / 2; ++t)
         {
             const int k = (lane >> 5) + 2 * t;
             const bool in = i < nb && k < nb;
-            const float v = i > k ? sC[i][k] : sC[i][GBS + k];
+            const float v = i > k ? sC[0][i][k] : sC[1][GBS - 1 - i][GBS - 1 - k];
             M.store(v, M.offset(in, r0 + i, r0 + k), 0);
         }
     }
@@ -602,10 +617,9 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
     {
         // diagonal blocks of the FP32 factors -> their inverses (used by every
         // triangular solve of the refinement)
-        const int64_t items = int64_t((n + GBS - 1) / GBS) * bc;
-        const int grid = int(items < 65536 * 16 ? items : 65536 * 16);
-        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(grid), dim3(64), 0, stream, n, LU32, n,
-                           info32, items);
+        const int64_t items = int64_t((n + GBS - 1) / GBS) * bc; // < 2^31: bc * 16 blocks
+        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(unsigned(items)), dim3(64), 0, stream,
+                           n, LU32, n, info32);
         RSA_LAUNCH_CHECK();
     }
     auto by_nr = [&](auto nmax) {
