@@ -71,7 +71,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     __shared__ TC sLtt[FNB * UB];
     __shared__ int spi[NMAX]; // logical row -> original row
     __shared__ short ssig[SMAX * NMAX]; // block t: logical row -> storage row
-    __shared__ TC scand[2 * NW * FNB];
+    __shared__ __align__(16) TC scand[2 * NW * FNB];
     __shared__ TC skey[2 * NW];
     __shared__ int slab[2 * NW];
 
@@ -335,29 +335,43 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 wave_argmax(key, lab);
                 if(label == lab)
                 {
+                    // whole 16-byte groups from the one holding column k
+                    // (aligned 16-byte stores; entries left of k are unused)
+                    constexpr int G = 16 / int(sizeof(TC));
 #pragma unroll
-                    for(int j = k; j < FNB; ++j)
-                        scand[(par * NW + w) * FNB + j] = rowv[j];
+                    for(int j = 0; j < FNB; ++j)
+                        if(j / G >= k / G)
+                            scand[(par * NW + w) * FNB + j] = rowv[j];
                     skey[par * NW + w] = key;
                     slab[par * NW + w] = lab;
                 }
                 __syncthreads();
+                // every wavefront's candidate read first, then a branch-free
+                // selection (ties to the lower row)
+                TC kws[NW];
+                int lws[NW];
+#pragma unroll
+                for(int ww = 0; ww < NW; ++ww)
+                {
+                    kws[ww] = skey[par * NW + ww];
+                    lws[ww] = slab[par * NW + ww];
+                }
                 int wb = 0;
-                TC bk = skey[par * NW];
-                int bl = slab[par * NW];
+                TC bk = kws[0];
+                int bl = lws[0];
 #pragma unroll
                 for(int ww = 1; ww < NW; ++ww)
                 {
-                    const TC kw = skey[par * NW + ww];
-                    const int lw = slab[par * NW + ww];
-                    if(kw > bk || (kw == bk && lw < bl))
-                    {
-                        bk = kw;
-                        bl = lw;
-                        wb = ww;
-                    }
+                    const bool better = kws[ww] > bk || (kws[ww] == bk && lws[ww] < bl);
+                    bk = better ? kws[ww] : bk;
+                    bl = better ? lws[ww] : bl;
+                    wb = better ? ww : wb;
                 }
-                const TC* prow = scand + (par * NW + wb) * FNB;
+                // the winner's row (its offset held in a vector register: as a
+                // scalar, every read of it waits for the previous one)
+                uint32_t poff = uint32_t((par * NW + wb) * FNB);
+                opaque(poff);
+                const TC* prow = scand + poff;
                 const TC pv = prow[k];
                 const bool nz = pv != TC(0);
                 if(!nz && zero_at == 0)
@@ -393,9 +407,21 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                         l *= TC(1) / pv;
                         rowv[k] = l;
                     }
+                    // 8-column chunks: a chunk's reads are in flight while the
+                    // previous chunk's fmas run
+                    constexpr int CH = 8;
 #pragma unroll
-                    for(int j = k + 1; j < FNB; ++j)
-                        rowv[j] = fma_t(-l, prow[j], rowv[j]);
+                    for(int c8 = (k + 1) / CH * CH; c8 < FNB; c8 += CH)
+                    {
+                        TC u[CH];
+#pragma unroll
+                        for(int q = 0; q < CH; ++q)
+                            u[q] = (c8 + q > k) ? prow[c8 + q] : TC(0);
+#pragma unroll
+                        for(int q = 0; q < CH; ++q)
+                            if(c8 + q > k)
+                                rowv[c8 + q] = fma_t(-l, u[q], rowv[c8 + q]);
+                    }
                 }
             }
         }
