@@ -37,6 +37,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 
 namespace rsa
 {
@@ -562,6 +563,25 @@ static void launch_ir(hipStream_t stream, int n, int nrhs, Batch<double> A, int 
                        fskip, berr, rs, cs, int64_t(n));
 }
 
+// Batches from this size on are solved in two halves on two streams.
+constexpr int SPLIT_MIN = 256;
+
+// One internal stream per device (default priority, non-blocking), created
+// on first use and kept for the life of the process; nullptr if it cannot be
+// created (the solve then runs on the call stream alone).
+static hipStream_t aux_stream()
+{
+    static std::mutex m;
+    static hipStream_t streams[64] = {};
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(m);
+    if(!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
+        streams[dev] = nullptr;
+    return streams[dev];
+}
+
 rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Batch<double> A,
                              int lda, Batch<double> B, int ldb, Batch<double> X, int ldx,
                              int* iter, int* info, double* berr, int bc, int itermax, void* work)
@@ -600,9 +620,58 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
             return st;                    \
     } while(0)
 
-    if(factor == 0)
-        RSA_TRY((lu_fused<double, float>(stream, n, A, lda, LU32, n, ipiv, n, info32, bc)));
-    else
+    // the per-matrix stages on the systems [k0, k0 + cnt): FP32 (or
+    // half-width) LU, diagonal-block inversion, refinement
+    auto sub = [&](auto batch, int64_t k) {
+        using BT = decltype(batch);
+        return batch.ptrs ? BT{nullptr, batch.ptrs + k, 0}
+                          : BT{batch.base + k * batch.stride, nullptr, batch.stride};
+    };
+    auto factor_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
+        if(factor == 0)
+            return lu_fused<double, float>(st_, n, sub(A, k0), lda, sub(LU32, k0), n,
+                                           ipiv + k0 * n, n, info32 + k0, cnt);
+        return lu_fused_lowp(st_, factor, n, sub(A, k0), lda, sub(LU32, k0), n, ipiv + k0 * n, n,
+                             info32 + k0, cnt, rs + k0 * n, cs + k0 * n, int64_t(n));
+    };
+    auto refine_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
+        // diagonal blocks of the FP32 factors -> their inverses (used by every
+        // triangular solve of the refinement)
+        const int64_t items = int64_t((n + GBS - 1) / GBS) * cnt; // < 2^31: cnt * 16 blocks
+        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(unsigned(items)), dim3(64), 0, st_, n,
+                           sub(LU32, k0), n, info32 + k0);
+        RSA_LAUNCH_CHECK();
+        const double* rs_ = rs ? rs + k0 * n : nullptr;
+        const double* cs_ = cs ? cs + k0 * n : nullptr;
+        double* berr_ = berr ? berr + k0 : nullptr;
+        auto by_nr = [&](auto nmax) {
+            constexpr int NMAX = decltype(nmax)::value;
+            auto go = [&](auto nr) {
+                constexpr int NR = decltype(nr)::value;
+                launch_ir<NMAX, NR>(st_, n, nrhs, sub(A, k0), lda, sub(LU32, k0), ipiv + k0 * n,
+                                    info32 + k0, sub(B, k0), ldb, sub(X, k0), ldx, itermax,
+                                    iter + k0, info + k0, fskip + k0, berr_, cnt, rs_, cs_);
+            };
+            if(nrhs <= 1)
+                go(std::integral_constant<int, 1>{});
+            else if constexpr(NMAX >= 512) // groups of 4 beyond 256 rows (8 would spill)
+                go(std::integral_constant<int, 4>{});
+            else if(nrhs <= 4)
+                go(std::integral_constant<int, 4>{});
+            else
+                go(std::integral_constant<int, 8>{});
+        };
+        if(n <= 128)
+            by_nr(std::integral_constant<int, 128>{});
+        else if(n <= 256)
+            by_nr(std::integral_constant<int, 256>{});
+        else
+            by_nr(std::integral_constant<int, 512>{});
+        RSA_LAUNCH_CHECK();
+        return rocblas_status_success;
+    };
+
+    if(factor != 0)
     {
         if(n <= 128)
             hipLaunchKernelGGL((equilibrate_kernel<128>), dim3(bc), dim3(128), 0, stream, n, A,
@@ -611,40 +680,49 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
             hipLaunchKernelGGL((equilibrate_kernel<256>), dim3(bc), dim3(256), 0, stream, n, A,
                                lda, rs, cs, int64_t(n));
         RSA_LAUNCH_CHECK();
-        RSA_TRY(lu_fused_lowp(stream, factor, n, A, lda, LU32, n, ipiv, n, info32, bc, rs, cs,
-                              int64_t(n)));
     }
+    hipStream_t aux = bc >= SPLIT_MIN ? aux_stream() : nullptr;
+    if(aux)
     {
-        // diagonal blocks of the FP32 factors -> their inverses (used by every
-        // triangular solve of the refinement)
-        const int64_t items = int64_t((n + GBS - 1) / GBS) * bc; // < 2^31: bc * 16 blocks
-        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(unsigned(items)), dim3(64), 0, stream,
-                           n, LU32, n, info32);
-        RSA_LAUNCH_CHECK();
+        // two halves at once, the second on an internal stream: each half's
+        // latency-bound LU and refinement fill the issue slots the other
+        // leaves idle; event fork and join keep the call stream-ordered and
+        // capturable in a graph
+        const int h = bc / 2;
+        hipEvent_t forked, joined;
+        if(hipEventCreateWithFlags(&forked, hipEventDisableTiming) != hipSuccess)
+            return rocblas_status_internal_error;
+        if(hipEventCreateWithFlags(&joined, hipEventDisableTiming) != hipSuccess)
+        {
+            (void)hipEventDestroy(forked);
+            return rocblas_status_internal_error;
+        }
+        bool ok = hipEventRecord(forked, stream) == hipSuccess
+                  && hipStreamWaitEvent(aux, forked, 0) == hipSuccess;
+        rocblas_status st1 = factor_range(stream, 0, h);
+        rocblas_status st2 = ok ? factor_range(aux, h, bc - h) : rocblas_status_internal_error;
+        if(st2 == rocblas_status_success)
+            st2 = refine_range(aux, h, bc - h);
+        // the join is recorded whatever happened on the internal stream, so
+        // the call stream never waits on an unrecorded event
+        ok = hipEventRecord(joined, aux) == hipSuccess;
+        if(st1 == rocblas_status_success)
+            st1 = refine_range(stream, 0, h);
+        ok = ok && hipStreamWaitEvent(stream, joined, 0) == hipSuccess;
+        (void)hipEventDestroy(forked);
+        (void)hipEventDestroy(joined);
+        if(st1 != rocblas_status_success)
+            return st1;
+        if(st2 != rocblas_status_success)
+            return st2;
+        if(!ok)
+            return rocblas_status_internal_error;
     }
-    auto by_nr = [&](auto nmax) {
-        constexpr int NMAX = decltype(nmax)::value;
-        if(nrhs <= 1)
-            launch_ir<NMAX, 1>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc, rs, cs);
-        else if(nrhs <= 4)
-            launch_ir<NMAX, 4>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc, rs, cs);
-        else if(NMAX < 512) // groups of 4 beyond 256 rows (8 would spill registers)
-            launch_ir<(NMAX < 512 ? NMAX : 256), 8>(stream, n, nrhs, A, lda, LU32, ipiv, info32,
-                                                    B, ldb, X, ldx, itermax, iter, info, fskip,
-                                                    berr, bc, rs, cs);
-        else
-            launch_ir<NMAX, 4>(stream, n, nrhs, A, lda, LU32, ipiv, info32, B, ldb, X, ldx,
-                               itermax, iter, info, fskip, berr, bc, rs, cs);
-    };
-    if(n <= 128)
-        by_nr(std::integral_constant<int, 128>{});
-    else if(n <= 256)
-        by_nr(std::integral_constant<int, 256>{});
     else
-        by_nr(std::integral_constant<int, 512>{});
-    RSA_LAUNCH_CHECK();
+    {
+        RSA_TRY(factor_range(stream, 0, bc));
+        RSA_TRY(refine_range(stream, 0, bc));
+    }
     // FP64 fallback for the systems that did not converge
     RSA_TRY((lu_fused<double, double>(stream, n, A, lda, LU64, n, ipiv, n, info, bc, fskip)));
     hipLaunchKernelGGL(ir_mark_singular_kernel, dim3(ceil_div(bc, 256)), dim3(256), 0, stream,

@@ -21,8 +21,9 @@ def dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+# batches of 256 and more are solved in two staggered halves on two streams
 @pytest.mark.parametrize("n,nrhs,bc", [(96, 16, 6), (128, 4, 5), (256, 8, 6), (200, 1, 4), (512, 2, 3),
-                                       (70, 3, 3)])
+                                       (70, 3, 3), (96, 3, 301)])
 def test_dsgesv_against_oracle(torch_gpu, handle, n, nrhs, bc):
     print(f"dsgesv n={n} nrhs={nrhs} bc={bc}", flush=True)
     A, B, kappa = wl.config3(bc=bc, n=n, nrhs=nrhs)
@@ -73,7 +74,7 @@ def test_dsgesv_fallback_paths(torch_gpu, handle):
 
 
 @pytest.mark.parametrize("factor", ["fp16", "bf16"])
-@pytest.mark.parametrize("n,nrhs,bc", [(96, 4, 6), (256, 8, 4), (300, 3, 2)])
+@pytest.mark.parametrize("n,nrhs,bc", [(96, 4, 6), (256, 8, 4), (300, 3, 2), (80, 2, 260)])
 def test_half_width_factor_refines_to_fp64(torch_gpu, handle, factor, n, nrhs, bc):
     """Equilibrated FP16/BF16-operand LU + FP64 refinement: well-conditioned
     systems converge by refinement to FP64 backward error; the result agrees
@@ -138,3 +139,45 @@ def test_half_width_factor_fallback_and_fp32_identity(torch_gpu, handle):
     torch_gpu.cuda.synchronize()
     assert torch_gpu.equal(X0, X32)
     np.testing.assert_array_equal(it0.cpu().numpy(), it32)
+
+
+def test_dsgesv_graph_capture_replays(torch_gpu):
+    """The mixed solve of a batch that takes the two-stream path, captured in a
+    graph on the handle's stream: every replay (with eager work and
+    synchronisations between replays) equals the eager call bit for bit."""
+    torch = torch_gpu
+    n, nrhs, bc = 96, 2, 320
+    A, B, _ = wl.config3(bc=bc, n=n, nrhs=nrhs, seed=123)
+    s = torch.cuda.Stream()
+    h = ra.Handle(s)
+    try:
+        Ad, Bd = dev(torch, A), dev(torch, B)
+        X = torch.zeros((bc, nrhs, n), dtype=torch.float64, device="cuda")
+        it = torch.zeros(bc, dtype=torch.int32, device="cuda")
+        info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+
+        def call():
+            ra.check("rocsolver_amd_dsgesv_strided_batched", h.value, n, nrhs, Ad.data_ptr(), n,
+                     n * n, Bd.data_ptr(), n, n * nrhs, X.data_ptr(), n, n * nrhs,
+                     it.data_ptr(), info.data_ptr(), 0, bc)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            call()  # eager: sizes the handle's workspace, reference result
+        torch.cuda.synchronize()
+        X0, it0, info0 = X.clone(), it.clone(), info.clone()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            call()
+        torch.cuda.synchronize()
+        for rep in range(3):
+            X.fill_(float(rep + 1))  # eager work between replays
+            it.fill_(-99)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(X, X0), f"replay {rep}"
+            assert torch.equal(it, it0) and torch.equal(info, info0), f"replay {rep}"
+        be = oracle.solve_backward_error(A, n, X0.cpu().numpy(), B)
+        assert be.max() <= 10 * n * EPS64
+    finally:
+        h.close()
