@@ -320,6 +320,20 @@ def bench_getrf32(args) -> int:
     t = float(np.mean(times))
     per = 2 * n * n * 8 + n * 4 + 4
     gbs = bc * per / t / 1e9
+    # achievable HBM rate on this box, for context: a 1 GiB device-to-device
+    # copy (1 GiB read + 1 GiB written), best of 5
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    copy_best = float("inf")
+    for _ in range(5):
+        e0.record(s)
+        dst.copy_(src)
+        e1.record(s)
+        torch.cuda.synchronize()
+        copy_best = min(copy_best, e0.elapsed_time(e1) * 1e-3)
+    copy_gbs = 2 * src.numel() * 8 / copy_best / 1e9
+    del src, dst
     cpu = None
     if not args.no_cpu:
         from oracle import lapack
@@ -348,7 +362,9 @@ def bench_getrf32(args) -> int:
         "roofline": {"kernel": "getrf_small_kernel<double,32,32,pivot> (team of 32 lanes per matrix)",
                      "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
                      "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
-                     "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4)},
+                     "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4),
+                     "device_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_device_copy": round(gbs / copy_gbs, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
