@@ -262,12 +262,12 @@ def bench_vbatched(args) -> int:
     return 0
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, summary: str = "traffic_mixed_gesv_n256_batch4096_nrhs8.md"):
     """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated against a 1 GiB
-    copy) of `kernel`, from the counter summary committed under profiles/; the
-    counters need their own profiler passes, so they are not collected here."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "traffic_mixed_gesv_n256_batch4096_nrhs8.md")
+    copy) of `kernel`, from the counter summary `summary` committed under
+    profiles/; the counters need their own profiler passes, so they are not
+    collected here."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", summary)
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
@@ -362,6 +362,8 @@ def bench_getrf32(args) -> int:
         "roofline": {"kernel": "getrf_small_kernel<double,32,32,pivot> (team of 32 lanes per matrix)",
                      "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
                      "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
+                     "traffic": measured_traffic("getrf_small_kernel<double,32,32,pivot>",
+                                                 "traffic_getrf_n32_batch131072.md"),
                      "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4),
                      "device_copy_gbs": round(copy_gbs, 1),
                      "frac_of_device_copy": round(gbs / copy_gbs, 4)},

@@ -3,8 +3,8 @@
 Run under rocprofv3 once per counter (FETCH_SIZE and WRITE_SIZE do not fit in
 one pass together with anything else):
 
-  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o fetch -- python3 -m rocsolver_amd.traffic
-  rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir> -o write -- python3 -m rocsolver_amd.traffic
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o fetch -- python3 -m rocsolver_amd.traffic [getrf32]
+  rocprofv3 --pmc WRITE_SIZE --output-format csv -d <dir> -o write -- python3 -m rocsolver_amd.traffic [getrf32]
   python3 -m rocsolver_amd.traffic --parse <dir>/.../fetch_counter_collection.csv \\
                                            <dir>/.../write_counter_collection.csv
 
@@ -13,8 +13,10 @@ The program first copies a 1 GiB device buffer through an elementwise kernel
 then launches, once each, on the metric workload's shapes (n = 256, batch
 4096, 8 right-hand sides, FP64 input): the FP32 LU of the FP64 batch
 (rocsolver_amd_dsgetrf_strided_batched) and the whole mixed-precision solve
-(rocsolver_amd_dsgesv_strided_batched).  The parser prints, per kernel, the
-counter values per dispatch scaled by the copy's factor.
+(rocsolver_amd_dsgesv_strided_batched); with the argument getrf32, instead,
+one plain FP64 rocsolver_dgetrf_strided_batched at n = 32 (lda 32, stride
+1024), batch 131072.  The parser prints, per kernel, the counter values per
+dispatch scaled by the copy's factor.
 """
 from __future__ import annotations
 
@@ -23,9 +25,10 @@ import sys
 
 COPY_BYTES = 1 << 30
 N, BC, NRHS = 256, 4096, 8
+N32, BC32 = 32, 131072
 
 
-def run() -> None:
+def run(workload: str = "mixed") -> None:
     import torch
 
     import rocsolver_amd as ra
@@ -37,6 +40,17 @@ def run() -> None:
     torch.cuda.synchronize()
     print("copy of 1 GiB done", flush=True)
     del src, dst
+    if workload == "getrf32":
+        g = torch.Generator(device="cuda").manual_seed(11)
+        A = torch.rand((BC32, N32, N32), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        ipiv = torch.zeros((BC32, N32), dtype=torch.int32, device="cuda")
+        info = torch.zeros(BC32, dtype=torch.int32, device="cuda")
+        h = ra.Handle()
+        ra.check("rocsolver_dgetrf_strided_batched", h.value, N32, N32, A.data_ptr(), N32,
+                 N32 * N32, ipiv.data_ptr(), N32, info.data_ptr(), BC32)
+        torch.cuda.synchronize()
+        print("dgetrf n=32 done", flush=True)
+        return
     g = torch.Generator(device="cuda").manual_seed(7)
     A = torch.randn((BC, N, N), dtype=torch.float64, device="cuda", generator=g)
     A += 4.0 * N ** 0.5 * torch.eye(N, dtype=torch.float64, device="cuda")
@@ -95,4 +109,4 @@ if __name__ == "__main__":
         import json
         print(json.dumps(parse(sys.argv[2], sys.argv[3]), indent=1))
     else:
-        run()
+        run(sys.argv[1] if len(sys.argv) > 1 else "mixed")
