@@ -47,6 +47,8 @@ def lib() -> ctypes.CDLL:
         for pre in ("d", "s"):
             getattr(L, f"orc_{pre}getf2").argtypes = [i, i, p, i, p, p]
             getattr(L, f"orc_{pre}getf2").restype = i
+            getattr(L, f"orc_{pre}getf2_npvt").argtypes = [i, i, p, i]
+            getattr(L, f"orc_{pre}getf2_npvt").restype = i
             getattr(L, f"orc_{pre}laswp").argtypes = [i, p, i, i, i, p, i]
             getattr(L, f"orc_{pre}laswp").restype = None
             getattr(L, f"orc_{pre}getrs").argtypes = [c, i, i, p, i, p, p, i]
@@ -104,6 +106,19 @@ def getf2(A: np.ndarray, m: int, with_gap: bool = False):
     if with_gap:
         return ipiv, info, gap[:, :k]
     return ipiv, info
+
+
+def getf2_npvt(A: np.ndarray, m: int):
+    """LU without pivoting of every matrix of a (bc, n, lda) batch, in place.
+
+    Returns info[bc] int32 (first zero diagonal pivot, 1-based, or 0)."""
+    A = _as_batch(A)
+    bc, n, lda = A.shape
+    info = np.zeros(bc, dtype=np.int32)
+    f = getattr(lib(), f"orc_{_prefix(A.dtype)}getf2_npvt")
+    for b in range(bc):
+        info[b] = f(m, n, _ptr(A[b]), lda)
+    return info
 
 
 def laswp(B: np.ndarray, k1: int, k2: int, ipiv: np.ndarray, incx: int) -> None:

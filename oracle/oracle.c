@@ -90,6 +90,42 @@
 DEF_GETF2(double, d, fabs, fma)
 DEF_GETF2(float, s, fabsf, fmaf)
 
+/* ------------------------------------------------------------- getf2_npvt
+ * No-pivoting LU, restating the reference small kernel
+ * library/src/specialized/roclapack_getf2_specialized_kernels.hpp:138-215
+ * (getf2_npvt_small_kernel, the path of m <= 64 batched npvt): the pivot is
+ * the diagonal entry; its reciprocal scales the column below it, and a zero
+ * pivot leaves the scale at 0 (:185-188), so the whole sub-column is
+ * multiplied by zero; info = first zero pivot; rank-1 update
+ * a -= l*u as one fma. */
+#define DEF_GETF2_NPVT(T, P, FMA)                                                       \
+    int orc_##P##getf2_npvt(int m, int n, T* A, int lda)                               \
+    {                                                                                   \
+        int info = 0;                                                                   \
+        int kmax = m < n ? m : n;                                                       \
+        for(int k = 0; k < kmax; ++k)                                                   \
+        {                                                                               \
+            T pv = A[IDX(k, k, lda)];                                                   \
+            T r = (T)0;                                                                 \
+            if(pv != (T)0)                                                              \
+                r = (T)1 / pv;                                                          \
+            else if(info == 0)                                                          \
+                info = k + 1;                                                           \
+            for(int i = k + 1; i < m; ++i)                                              \
+                A[IDX(i, k, lda)] *= r;                                                 \
+            for(int j = k + 1; j < n; ++j)                                              \
+            {                                                                           \
+                T u = A[IDX(k, j, lda)];                                                \
+                for(int i = k + 1; i < m; ++i)                                          \
+                    A[IDX(i, j, lda)] = FMA(-A[IDX(i, k, lda)], u, A[IDX(i, j, lda)]);  \
+            }                                                                           \
+        }                                                                               \
+        return info;                                                                    \
+    }
+
+DEF_GETF2_NPVT(double, d, fma)
+DEF_GETF2_NPVT(float, s, fmaf)
+
 /* ------------------------------------------------------------------ laswp */
 #define DEF_LASWP(T, P)                                                                 \
     void orc_##P##laswp(int n, T* A, int lda, int k1, int k2, const int* ipiv, int incx) \

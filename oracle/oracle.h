@@ -26,6 +26,12 @@ extern "C" {
 int orc_dgetf2(int m, int n, double* A, int lda, int* ipiv, double* gap);
 int orc_sgetf2(int m, int n, float* A, int lda, int* ipiv, double* gap);
 
+/* LU without pivoting (the reference npvt small kernel): a zero diagonal
+ * pivot sets info (first, 1-based) and multiplies the column below it by
+ * zero.  Returns info. */
+int orc_dgetf2_npvt(int m, int n, double* A, int lda);
+int orc_sgetf2_npvt(int m, int n, float* A, int lda);
+
 /* Row interchanges, reference laswp semantics (k1, k2 1-based, incx = +-1). */
 void orc_dlaswp(int n, double* A, int lda, int k1, int k2, const int* ipiv, int incx);
 void orc_slaswp(int n, float* A, int lda, int k1, int k2, const int* ipiv, int incx);

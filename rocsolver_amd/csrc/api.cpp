@@ -282,10 +282,6 @@ rocblas_status vbatched_impl(rocblas_handle handle, int op, rocblas_operation tr
 {
     if(!handle)
         return rocblas_status_invalid_handle;
-    // getrf alone is switched off: its first GPU run was followed by a hang
-    // whose cause has not been found (getrs and gesv are on)
-    if(op == 0)
-        return rocblas_status_not_implemented;
     bool tr = false;
     if(op == 1)
     {

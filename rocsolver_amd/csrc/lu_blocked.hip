@@ -125,13 +125,16 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
             }
         }
         __syncthreads();
-        // scale the column and update the rest of the panel
+        // scale the column and update the rest of the panel.  A zero pivot
+        // leaves the column alone with pivoting (every candidate is zero) and
+        // multiplies it by zero without (the reference npvt kernels keep the
+        // scale at 0: roclapack_getf2_specialized_kernels.hpp:185-188,427-430)
         const int jend = j0 + jb;
-        const T rcp = nz ? T(1) / pv : T(1);
+        const T rcp = nz ? T(1) / pv : T(0);
         for(int i = c + 1 + tid; i < m; i += PANEL_THREADS)
         {
             T l = at(i, c);
-            if(nz)
+            if(nz || !PIVOT)
             {
                 l *= rcp;
                 at(i, c) = l;

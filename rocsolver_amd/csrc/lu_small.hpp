@@ -80,8 +80,14 @@ __device__ __forceinline__ void team_lu(T (&r)[NC], T* prow, int m, int n, bool 
                 label = plab;
             if(live && label > k)
             {
+                // a zero pivot: with pivoting every candidate is zero and the
+                // column stays; without, the reference npvt small kernel
+                // multiplies the column by its scale of 0
+                // (roclapack_getf2_specialized_kernels.hpp:185-201)
                 if(nz)
                     r[k] *= T(1) / pv;
+                else if(!PIVOT)
+                    r[k] *= T(0);
                 const T l = r[k];
 #pragma unroll
                 for(int j = k + 1; j < NC; ++j)

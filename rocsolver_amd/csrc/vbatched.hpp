@@ -161,6 +161,51 @@ __device__ __forceinline__ void vb_big(const VbArgs<T>& a, int b)
     }
 }
 
+// One pack of small matrices: the bucket's team size picks the instantiation.
+template <typename T, int OP>
+__device__ __forceinline__ void vb_small_item(const VbArgs<T>& a, const VbTuning& tun,
+                                              const int* __restrict__ order, const VbItem& item,
+                                              T (*prow)[64], T (*pb)[VB_NR], int (*perm)[64],
+                                              int (*spiv)[64])
+{
+    const int ts = tun.team[item.kind];
+#define RSA_VB_PACK(NC_, TS_) vb_small_pack<T, OP, NC_, TS_>(a, order, item, prow, pb, perm, spiv)
+    switch(item.kind)
+    {
+    case 0:
+        if(ts == 16)
+            RSA_VB_PACK(8, 16);
+        else if(ts == 32)
+            RSA_VB_PACK(8, 32);
+        else
+            RSA_VB_PACK(8, 64);
+        break;
+    case 1:
+        if(ts == 16)
+            RSA_VB_PACK(16, 16);
+        else if(ts == 32)
+            RSA_VB_PACK(16, 32);
+        else
+            RSA_VB_PACK(16, 64);
+        break;
+    case 2:
+        if(ts == 32)
+            RSA_VB_PACK(24, 32);
+        else
+            RSA_VB_PACK(24, 64);
+        break;
+    case 3:
+        if(ts == 32)
+            RSA_VB_PACK(32, 32);
+        else
+            RSA_VB_PACK(32, 64);
+        break;
+    case 4: RSA_VB_PACK(48, 64); break;
+    default: RSA_VB_PACK(64, 64); break;
+    }
+#undef RSA_VB_PACK
+}
+
 template <typename T, int OP>
 __global__ __launch_bounds__(VB_THREADS, 2) void vb_work_kernel(VbArgs<T> a, VbTuning tun,
                                                                 const int* __restrict__ order,
@@ -175,58 +220,31 @@ __global__ __launch_bounds__(VB_THREADS, 2) void vb_work_kernel(VbArgs<T> a, VbT
     const int w = threadIdx.x >> 6;
     const int wp = OP == VB_GETRS ? w : 0;
     const int nitems = ctrl[0];
+    // Thread 0 claims the next item at the END of an iteration, and a barrier
+    // follows the claim, so the loop's back-edge is reached by every lane
+    // only through that barrier.  With the claim at the top of the loop the
+    // compiler may merge the divergent `threadIdx.x == 0` branch ending the
+    // previous item (the info store of a large getrf) with the claim's, and
+    // structure the result as an inner loop that the other lanes re-enter at
+    // the barrier without waiting for lane 0's claim: they read the old item
+    // and factor it again, forever.
+    if(threadIdx.x == 0)
+        s_item = atomicAdd(&ctrl[1], 1);
+    __syncthreads();
     for(;;)
     {
-        if(threadIdx.x == 0)
-            s_item = atomicAdd(&ctrl[1], 1);
-        __syncthreads();
-        const int it = s_item;
-        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads(); // every wave has read s_item before it is rewritten
         if(it >= nitems)
             break;
         const VbItem item = items[it];
         if(item.kind == VB_KIND_BIG)
-        {
             vb_big<T, OP>(a, order[item.first]);
-            continue;
-        }
-        const int ts = tun.team[item.kind];
-#define RSA_VB_PACK(NC_, TS_)                                                                   \
-    vb_small_pack<T, OP, NC_, TS_>(a, order, item, prow[w], pb[w], perm[wp], spiv[wp])
-        switch(item.kind)
-        {
-        case 0:
-            if(ts == 16)
-                RSA_VB_PACK(8, 16);
-            else if(ts == 32)
-                RSA_VB_PACK(8, 32);
-            else
-                RSA_VB_PACK(8, 64);
-            break;
-        case 1:
-            if(ts == 16)
-                RSA_VB_PACK(16, 16);
-            else if(ts == 32)
-                RSA_VB_PACK(16, 32);
-            else
-                RSA_VB_PACK(16, 64);
-            break;
-        case 2:
-            if(ts == 32)
-                RSA_VB_PACK(24, 32);
-            else
-                RSA_VB_PACK(24, 64);
-            break;
-        case 3:
-            if(ts == 32)
-                RSA_VB_PACK(32, 32);
-            else
-                RSA_VB_PACK(32, 64);
-            break;
-        case 4: RSA_VB_PACK(48, 64); break;
-        default: RSA_VB_PACK(64, 64); break;
-        }
-#undef RSA_VB_PACK
+        else
+            vb_small_item<T, OP>(a, tun, order, item, prow[w], pb[w], perm[wp], spiv[wp]);
+        if(threadIdx.x == 0)
+            s_item = atomicAdd(&ctrl[1], 1);
+        __syncthreads();
     }
 }
 

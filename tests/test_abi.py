@@ -107,9 +107,8 @@ def test_vbatched_argument_checks(p):
                    DUMMY, DUMMY, DUMMY, None, -1) == 11
     assert ra.call(f"rocsolver_amd_{p}getrs_vbatched", DUMMY, 111, DUMMY, DUMMY, DUMMY, DUMMY,
                    DUMMY, DUMMY, DUMMY, None, -1) == 4
-    # getrf alone is switched off for now (rocblas_status_not_implemented = 2)
     assert ra.call(f"rocsolver_amd_{p}getrf_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
-                   -1) == 2
+                   -1) == 4
     assert ra.call(f"rocsolver_amd_{p}gesv_vbatched", DUMMY, DUMMY, DUMMY, DUMMY, DUMMY, DUMMY,
                    DUMMY, DUMMY, DUMMY, -1) == 4
 

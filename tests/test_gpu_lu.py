@@ -301,3 +301,30 @@ def test_config2_full_batch_properties(torch_gpu, handle):
     As = A[idx].copy()
     ip_o, in_o = oracle.getf2(As, n)
     np.testing.assert_array_equal(host(ip)[idx], ip_o)
+
+
+def test_config2_ipiv_and_solve_full_batch(torch_gpu, handle):
+    """n=128 benchmark workload (bc=16384, 4 RHS): ipiv and info exactly as
+    the oracle on every 8th matrix (2048 matrices), then the 4-RHS getrs of
+    the whole batch with FP64 backward error <= 10 n eps checked on the GPU."""
+    torch = torch_gpu
+    A, B = wl.config2()
+    n, bc = 128, A.shape[0]
+    Ad, Bd = dev(torch, A), dev(torch, B)
+    ip, info = ra.getrf(handle, Ad)
+    ra.getrs(handle, Ad, ip, Bd)
+    torch.cuda.synchronize()
+    assert (host(info) == 0).all()
+    idx = np.arange(0, bc, 8)
+    As = A[idx].copy()
+    ip_o, in_o = oracle.getf2(As, n)
+    np.testing.assert_array_equal(host(ip)[idx], ip_o)
+    np.testing.assert_array_equal(host(info)[idx], in_o)
+    A0 = dev(torch, A)
+    B0 = dev(torch, B)
+    AX = torch.matmul(Bd, A0)
+    rn = (B0 - AX).abs().amax(dim=2)
+    an = A0.abs().sum(dim=1).amax(dim=1)
+    den = an[:, None] * Bd.abs().amax(dim=2) + B0.abs().amax(dim=2)
+    be = float((rn / den).max())
+    assert be <= 10 * n * EPS[np.float64], be

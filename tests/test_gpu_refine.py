@@ -181,3 +181,47 @@ def test_dsgesv_graph_capture_replays(torch_gpu):
         assert be.max() <= 10 * n * EPS64
     finally:
         h.close()
+
+
+def gpu_backward_error(torch, A, X, B):
+    """max over right-hand sides of ||b - A x||_inf / (||A||_inf ||x||_inf +
+    ||b||_inf) per system, in FP64 on the GPU.  A (bc, n, n) column-major per
+    matrix (A[b, j, i] = a_ij), X and B (bc, nrhs, n)."""
+    AX = torch.matmul(X, A)  # (A x_c)_i = sum_j A[b, j, i] X[b, c, j]
+    rn = (B - AX).abs().amax(dim=2)
+    an = A.abs().sum(dim=1).amax(dim=1)
+    den = an[:, None] * X.abs().amax(dim=2) + B.abs().amax(dim=2)
+    return (rn / den).amax(dim=1)
+
+
+def test_dsgesv_metric_batch(torch_gpu, handle):
+    """The benchmark configuration itself: n=256, batch 4096, 8 right-hand
+    sides, kappa log-uniform in [1e2, 1e6] (the batch is large enough for the
+    two-stream split).  Every system reaches FP64 backward error <= 10 n eps64
+    (checked on the GPU for all 4096); info exactly and the outcome class per
+    system as the oracle's dsgesv on every 64th system."""
+    torch = torch_gpu
+    n, nrhs, bc = 256, 8, 4096
+    print(f"dsgesv metric batch n={n} nrhs={nrhs} bc={bc}", flush=True)
+    A, B, kappa = wl.config3_torch(bc=bc, n=n, nrhs=nrhs)
+    X, it, info, berr = ra.dsgesv(handle, A, B)
+    torch.cuda.synchronize()
+    be = gpu_backward_error(torch, A, X, B)
+    assert float(be.max()) <= 10 * n * EPS64, float(be.max())
+    assert (info == 0).all()
+    itc = it.cpu().numpy()
+    idx = np.arange(0, bc, 64)
+    Ah = A[idx].cpu().numpy()
+    Bh = B[idx].cpu().numpy()
+    Xo, it_o, info_o = oracle.dsgesv(Ah, n, Bh)
+    np.testing.assert_array_equal(info.cpu().numpy()[idx], info_o)
+    np.testing.assert_array_equal(itc[idx] >= 0, it_o >= 0)
+    conv = it_o >= 0
+    assert np.all(np.abs(itc[idx][conv] - it_o[conv]) <= 2)
+    # the per-system FP64 forward error against the oracle's solution
+    Xs = X[idx].cpu().numpy()
+    for k, b in enumerate(idx):
+        fe = np.abs(Xs[k] - Xo[k]).max() / np.abs(Xo[k]).max()
+        assert fe <= kappa[b].item() * n * EPS64 * 10
+    print(f"iterations: {np.bincount(itc[itc >= 0]).tolist()}, fallbacks {(itc < 0).sum()}, "
+          f"max backward error {float(be.max()):.2e}", flush=True)

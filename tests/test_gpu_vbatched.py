@@ -96,8 +96,6 @@ def test_getrs_vbatched_equals_fixed_size(torch_gpu, handle, dtype, trans):
                                       err_msg=f"n={n[b]}")
 
 
-@pytest.mark.skip(reason="getrf_vbatched is switched off (not_implemented): the first GPU run of "
-                         "this test was followed by a hang whose cause has not been found")
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_getrf_getrs_vbatched_equal_fixed_size(torch_gpu, handle, dtype, trans):
@@ -210,3 +208,42 @@ def test_vbatched_tuning_does_not_change_results(torch_gpu, handle):
     for o in out[1:]:
         for x, y in zip(out[0], o):
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("op", ["gesv", "getrf"])
+def test_vbatched_graph_replays_equal_eager(torch_gpu, op):
+    """The plan + persistent-kernel sequence captured in a graph (workspace
+    allocated by an eager call first) and replayed several times, with eager
+    work and synchronisations between replays: every replay equals the eager
+    result bit for bit."""
+    torch = torch_gpu
+    n, nrhs, As, Bs = make(21, ORDERS, "float64")
+    vb = load(torch, n, nrhs, As, Bs, "float64")
+    A0, B0 = vb.A.clone(), vb.B.clone()
+    h = ra.Handle()
+    try:
+        run = (lambda: vb.gesv(h)) if op == "gesv" else (lambda: vb.getrf(h))
+        run()
+        torch.cuda.synchronize()
+        ref = (vb.A.clone(), vb.B.clone(), vb.ipiv.clone(), vb.info.clone())
+        g = torch.cuda.CUDAGraph()
+        vb.A.copy_(A0)
+        vb.B.copy_(B0)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            h.set_stream(torch.cuda.current_stream())
+            run()
+        h.set_stream(torch.cuda.current_stream())
+        for rep in range(4):
+            vb.A.copy_(A0)
+            vb.B.copy_(B0)
+            vb.ipiv.zero_()
+            vb.info.fill_(-7)
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for x, y in zip((vb.A, vb.B, vb.ipiv, vb.info), ref):
+                assert torch.equal(x, y), f"replay {rep}"
+            vb.A.mul_(1.0)  # eager work between replays
+    finally:
+        h.close()

@@ -331,3 +331,100 @@ def test_residual_and_norms_against_numpy():
         np.testing.assert_allclose(R[b].T, B[b].T - M @ X[b].T, atol=1e-12)
         assert np.isclose(oracle.norm_inf(A[b:b + 1], n, n)[0], np.abs(M).sum(axis=1).max())
         assert np.isclose(oracle.norm_fro(A[b:b + 1], n, n)[0], np.linalg.norm(M))
+
+
+# ------------------------------------------------------------- getf2_npvt
+@pytest.mark.parametrize("m,n", [(1, 1), (8, 8), (32, 32), (50, 30), (30, 50), (100, 100)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_getf2_npvt_equals_pivoting_lu_when_no_interchange(m, n, dtype):
+    """On strictly diagonally dominant columns partial pivoting picks the
+    diagonal every step, so the no-pivot restatement must reproduce the
+    pivoting LU bit for bit with ipiv = identity."""
+    g = np.random.default_rng(m + 7 * n)
+    A = g.uniform(-1, 1, size=(2, n, m)).astype(dtype)
+    for j in range(min(m, n)):
+        A[:, j, j] += 2 * m * np.sign(A[:, j, j] + 0.5)
+    Ap, An = A.copy(), A.copy()
+    ipiv, info_p = oracle.getf2(Ap, m)
+    info_n = oracle.getf2_npvt(An, m)
+    np.testing.assert_array_equal(ipiv, np.tile(np.arange(1, min(m, n) + 1), (2, 1)))
+    np.testing.assert_array_equal(An, Ap)
+    np.testing.assert_array_equal(info_n, info_p)
+
+
+def test_getf2_npvt_zero_pivot_closed_forms():
+    """A zero diagonal pivot: info = its position, the column below it is
+    multiplied by the scale 0, the factorisation continues."""
+    # [[0, 1], [1, 1]] (row-major) -> L21 = 0, U = [[0, 1], [0, 1]]
+    A = np.array([[[0.0, 1.0], [1.0, 1.0]]])  # column-major: col 0 = (0, 1)
+    assert oracle.getf2_npvt(A, 2)[0] == 1
+    np.testing.assert_array_equal(A[0], [[0.0, 0.0], [1.0, 1.0]])
+    # 3x3 with a zero second pivot after an exact first step:
+    # rows (2, 4, 2), (1, 2, 5), (3, 7, 1):  l = (1/2, 3/2); rows 2, 3 of the
+    # trailing block become (0, 4) and (1, -2); pivot 2 is 0 -> info 2, its
+    # column below scaled by 0; u33 = -2 - 0 * 4 = -2
+    R = np.array([[2.0, 4, 2], [1, 2, 5], [3, 7, 1]])
+    A = np.ascontiguousarray(R.T)[None].copy()
+    assert oracle.getf2_npvt(A, 3)[0] == 2
+    LU = A[0].T
+    np.testing.assert_array_equal(LU, [[2, 4, 2], [0.5, 0, 4], [1.5, 0, -2]])
+
+
+def test_getf2_npvt_singular_columns_info_vs_lapack():
+    """The reference npvt test's singular matrices (diagonally dominant, three
+    zeroed columns; testing_getf2_getrf_npvt.hpp:113-149): info equals host
+    LAPACK getrf's, the quantity that test compares exactly."""
+    if not lapack.available():
+        pytest.skip("no host LAPACK")
+    n = 40
+    bc = 8
+    A = wl.reference_gesv(n, 1, n, n, bc, singular=True)[0]
+    An = A.copy()
+    info_n = oracle.getf2_npvt(An, n)
+    _, info_l = lapack.getrf(A.copy(), n)
+    np.testing.assert_array_equal(info_n, info_l)
+    assert (info_n[[bc // 4, bc // 2, bc - 1]] > 0).all()
+
+
+# ------------------------------------------------------ LU residual checker
+def test_lu_residual_reports_a_known_perturbation():
+    """orc_dlu_residual on exact integer factors: 0 for the true LU, and the
+    exact Frobenius norm of the change in L*U after one entry is perturbed."""
+    n = 5
+    L = np.eye(n)
+    L[np.tril_indices(n, -1)] = [1, -2, 3, 1, 0, -1, 2, 1, -3, 1]
+    U = np.triu(np.arange(1, n * n + 1, dtype=np.float64).reshape(n, n) % 7 + 1)
+    perm = [2, 0, 4, 1, 3]
+    LUrm = L @ U  # = P A
+    A_rm = np.empty_like(LUrm)
+    # ipiv that realises the permutation by successive interchanges
+    ipiv = np.zeros(n, dtype=np.int32)
+    rows = list(range(n))
+    target = perm[:]
+    for k in range(n):
+        p = rows.index(target[k])
+        ipiv[k] = p + 1
+        rows[k], rows[p] = rows[p], rows[k]
+    for k in range(n):
+        A_rm[rows[k]] = LUrm[k]
+    packed = np.tril(L, -1) + U
+    A0 = np.ascontiguousarray(A_rm.T)[None]
+    LU = np.ascontiguousarray(packed.T)[None]
+    lib = oracle.lib()
+    r0 = lib.orc_dlu_residual(n, n, A0.ctypes.data, n, LU.ctypes.data, n, ipiv.ctypes.data)
+    assert r0 == 0.0
+    # U[n-1, n-1] += 0.5 changes only (L U)[n-1, n-1], by 0.5
+    LU2 = LU.copy()
+    LU2[0, n - 1, n - 1] += 0.5
+    r1 = lib.orc_dlu_residual(n, n, A0.ctypes.data, n, LU2.ctypes.data, n, ipiv.ctypes.data)
+    assert r1 == 0.5
+    # L[3, 1] += 0.25 changes row 3 of L U by 0.25 * U[1, :]
+    LU3 = LU.copy()
+    LU3[0, 1, 3] += 0.25
+    r2 = lib.orc_dlu_residual(n, n, A0.ctypes.data, n, LU3.ctypes.data, n, ipiv.ctypes.data)
+    assert np.isclose(r2, 0.25 * np.linalg.norm(U[1]), rtol=1e-15)
+    # a wrong pivot sequence is seen too
+    ip2 = ipiv.copy()
+    ip2[0] = 1 if ipiv[0] != 1 else 2
+    r3 = lib.orc_dlu_residual(n, n, A0.ctypes.data, n, LU.ctypes.data, n, ip2.ctypes.data)
+    assert r3 > 1.0
