@@ -350,6 +350,23 @@ rocblas_status rocsolver_amd_dgesv_vbatched(rocblas_handle handle, const rocblas
                                             double* const B[], const rocblas_int* ldb,
                                             rocblas_int* info, const rocblas_int batch_count);
 
+/* rocsolver_amd_dgesv_vbatched with no allocation of any kind.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The work queue lives in a static device array of the
+ * library (1.25 MiB per GPU, held until the process ends) instead of handle
+ * workspace; a size query reports 0 bytes.  batch_count above 65536 returns
+ * rocblas_status_invalid_size.  Results are those of
+ * rocsolver_amd_dgesv_vbatched.  Every call on one GPU shares that GPU's
+ * array: calls that overlap in time on the same GPU (different streams or
+ * threads) race and must be serialised by the caller. */
+rocblas_status rocsolver_amd_dgesv_vbatched_noalloc(rocblas_handle handle, const rocblas_int* n,
+                                                   const rocblas_int* nrhs, double* const A[],
+                                                   const rocblas_int* lda,
+                                                   rocblas_int* const ipiv[], double* const B[],
+                                                   const rocblas_int* ldb, rocblas_int* info,
+                                                   const rocblas_int batch_count);
+
 /* Schedule table of the variable-size kernel (process-wide), 13 integers:
  * team[6] (lanes per matrix for orders <= 8, 16, 24, 32, 48, 64: 16, 32 or
  * 64, at least the order), pack[6] (rounds of teams per work item, 1..64),

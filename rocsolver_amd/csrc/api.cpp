@@ -310,6 +310,31 @@ rocblas_status vbatched_impl(rocblas_handle handle, int op, rocblas_operation tr
                        static_cast<void*>(mem));
 }
 
+// gesv of a variable-size FP64 batch with the work queue in the library's
+// static device array instead of handle workspace: no allocation at all.
+rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, const int* nrhs,
+                                           double* const* A, const int* lda, int* const* ipiv,
+                                           double* const* B, const int* ldb, int* info, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(bc < 0 || bc > VB_NOALLOC_MAX_BATCH)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && bc > 0 && (!n || !lda || !A || !ipiv || !nrhs || !B || !ldb || !info))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+    if(bc == 0)
+        return rocblas_status_success;
+    void* work = vbatched_noalloc_scratch();
+    if(!work)
+        return rocblas_status_internal_error;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    return vbatched<double>(stream, 2, false, n, nrhs, A, lda, ipiv, B, ldb, info, bc, work);
+}
+
 } // namespace
 
 #define RSA_GETRF_FAMILY(P, T)                                                                \
@@ -526,6 +551,16 @@ RSA_ADJOINT(d, double)
     }
 RSA_VBATCHED(s, float)
 RSA_VBATCHED(d, double)
+
+rocblas_status rocsolver_amd_dgesv_vbatched_noalloc(rocblas_handle handle, const rocblas_int* n,
+                                                   const rocblas_int* nrhs, double* const A[],
+                                                   const rocblas_int* lda,
+                                                   rocblas_int* const ipiv[], double* const B[],
+                                                   const rocblas_int* ldb, rocblas_int* info,
+                                                   const rocblas_int batch_count)
+{
+    return dgesv_vbatched_noalloc_impl(handle, n, nrhs, A, lda, ipiv, B, ldb, info, batch_count);
+}
 
 rocblas_status rocsolver_amd_vbatched_set_tuning(const rocblas_int* params, const rocblas_int len)
 {

@@ -101,8 +101,24 @@ constexpr int VB_TUNING_LEN = 2 * VB_BUCKETS + 1;
 
 VbTuning vbatched_tuning(); // current process-wide table
 bool vbatched_set_tuning(const int* params, int len);
-size_t vbatched_workspace_bytes(int bc);
+// Workspace of the variable-size routines: the order list (one int per
+// matrix), the work items (16 bytes each, at most one per matrix plus one
+// partial pack per bucket) and the two queue counters, each 256-byte aligned.
+constexpr size_t vbatched_workspace_bytes(int bc)
+{
+    return (sizeof(int) * size_t(bc) + 255) / 256 * 256
+           + (16 * (size_t(bc) + VB_BUCKETS) + 255) / 256 * 256 + 256;
+}
 int device_cu_count();
+
+// Largest batch of the entry point that allocates nothing
+// (rocsolver_amd_dgesv_vbatched_noalloc): its work queue lives in a static
+// device array of vbatched_workspace_bytes(VB_NOALLOC_MAX_BATCH) bytes
+// (1.25 MiB) per GPU.
+constexpr int VB_NOALLOC_MAX_BATCH = 65536;
+// Device address of that array on the current device (looked up once per
+// device), or null if the lookup failed.
+void* vbatched_noalloc_scratch();
 
 // op: 0 getrf, 1 getrs, 2 gesv.  info may be null for getrs.
 template <typename T>

@@ -61,11 +61,29 @@ int device_cu_count()
     return cache[dev];
 }
 
-size_t vbatched_workspace_bytes(int bc)
+static_assert(sizeof(VbItem) == 16, "work items are 16 bytes");
+
+// Work queue of rocsolver_amd_dgesv_vbatched_noalloc, one copy per GPU for
+// the life of the process.  Host code never uses this name as an address:
+// the device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) char scratch[vbatched_workspace_bytes(VB_NOALLOC_MAX_BATCH)];
+
+void* vbatched_noalloc_scratch()
 {
-    const size_t order = (sizeof(int) * size_t(bc) + 255) / 256 * 256;
-    const size_t items = (sizeof(VbItem) * (size_t(bc) + VB_BUCKETS) + 255) / 256 * 256;
-    return order + items + 256;
+    static std::mutex m;
+    static void* ptr[64] = {nullptr};
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(m);
+    if(!ptr[dev])
+    {
+        void* p = nullptr;
+        if(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return nullptr;
+        ptr[dev] = p;
+    }
+    return ptr[dev];
 }
 
 // Matrix b: -2 invalid arguments, -1 nothing to compute, 0..5 small bucket,
