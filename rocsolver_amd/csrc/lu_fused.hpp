@@ -94,40 +94,57 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         const int jb = (n - c0) < FNB ? (n - c0) : FNB;
         acc_t acc[4][2];
 
-        // ---- 1. gather block column s into the accumulators (logical order),
-        //         KC columns per round: thread tid loads original row tid
-#pragma unroll
-        for(int kc = 0; kc < NKC; ++kc)
+        // ---- 1. gather block column s into the accumulators (logical order):
+        //         thread tid loads original row tid.  The loads of the second
+        //         KC columns are issued as the first KC are consumed, so both
+        //         LDS rounds are served by about one memory round trip while
+        //         at most KC loads are held in registers.
         {
+            uint32_t off = Ain.offset(tid < n, tid, c0);
+            const uint32_t step = tid < n ? Ain.ld_bytes : 0u;
+            TI a0[KC], a1[KC];
+#pragma unroll
+            for(int j = 0; j < KC; ++j)
             {
-                uint32_t off = Ain.offset(tid < n, tid, c0 + KC * kc);
-                const uint32_t step = tid < n ? Ain.ld_bytes : 0u;
-#pragma unroll
-                for(int j = 0; j < KC; ++j)
-                {
-                    const int col = c0 + KC * kc + j;
-                    const TI a = Ain.load(off, 0); // 0 past row or column n
-                    off += step;
-                    opaque(off);
-                    sbuf[j * NMAX + tid]
-                        = rs ? TC(double(a) * rs[tid < n ? tid : 0] * cs[col < n ? col : 0])
-                             : TC(a);
-                }
+                a0[j] = Ain.load(off, 0); // 0 past row or column n
+                off += step;
+                opaque(off);
             }
-            __syncthreads();
+            auto put = [&](int kc, int j, TI v) {
+                const int col = c0 + KC * kc + j;
+                sbuf[j * NMAX + tid]
+                    = rs ? TC(double(v) * rs[tid < n ? tid : 0] * cs[col < n ? col : 0]) : TC(v);
+            };
+            auto take = [&](int kc) {
+                __syncthreads();
 #pragma unroll
-            for(int ct = 0; ct < 2; ++ct)
-                if(ct / (2 / NKC) == kc)
+                for(int ct = 0; ct < 2; ++ct)
+                    if(ct / (2 / NKC) == kc)
 #pragma unroll
-                    for(int rt = 0; rt < 4; ++rt)
+                        for(int rt = 0; rt < 4; ++rt)
 #pragma unroll
-                        for(int r = 0; r < 4; ++r)
-                        {
-                            const int i = 64 * w + 16 * rt + M::row(lane, r);
-                            const int j = 16 * ct - KC * kc + (lane & 15);
-                            acc[rt][ct][r] = sbuf[j * NMAX + spi[i]];
-                        }
-            __syncthreads();
+                            for(int r = 0; r < 4; ++r)
+                            {
+                                const int i = 64 * w + 16 * rt + M::row(lane, r);
+                                const int j = 16 * ct - KC * kc + (lane & 15);
+                                acc[rt][ct][r] = sbuf[j * NMAX + spi[i]];
+                            }
+                __syncthreads();
+            };
+            static_assert(NKC == 2, "two LDS rounds per block column");
+#pragma unroll
+            for(int j = 0; j < KC; ++j)
+            {
+                put(0, j, a0[j]);
+                a1[j] = Ain.load(off, 0);
+                off += step;
+                opaque(off);
+            }
+            take(0);
+#pragma unroll
+            for(int j = 0; j < KC; ++j)
+                put(1, j, a1[j]);
+            take(1);
         }
 
         // ---- 2. left-looking update with every factored block column t < s.
@@ -442,28 +459,48 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         __syncthreads();
     }
 
-    // ---- move every L21_t into the final row order
-    for(int t = 0; t + 1 < S; ++t)
+    // ---- move every L21_t into the final row order: row i of block column
+    // t takes storage row ssig[t][i].  Each thread loads its source row (32
+    // entries), a barrier orders every load of the block column before any
+    // store into it, and the next block column's loads are issued before
+    // this one's stores, so the pass costs about one memory round trip.
+    if(S > 1)
     {
-        const int r1 = (t + 1) * FNB;
+        auto load_block = [&](int t, TC (&x)[FNB]) {
+            const int r1 = (t + 1) * FNB;
+            const bool act = tid >= r1 && tid < n;
+            uint32_t off = Aout.offset(act, act ? int(ssig[t * NMAX + tid]) : 0, t * FNB);
+            const uint32_t step = act ? Aout.ld_bytes : 0u;
 #pragma unroll
-        for(int ct = 0; ct < 2; ++ct)
+            for(int j = 0; j < FNB; ++j)
+            {
+                x[j] = Aout.load(off, 0);
+                off += step;
+                opaque(off);
+            }
+        };
+        TC cur[FNB];
+        load_block(0, cur);
+        for(int t = 0; t + 1 < S; ++t)
         {
-            const int cb = t * FNB + FHC * ct;
-            for(int e = tid; e < FHC * NMAX; e += NMAX)
+            __syncthreads(); // every thread's loads of block column t are done
+            TC nxt[FNB];
+            if(t + 2 < S)
+                load_block(t + 1, nxt);
+            const int r1 = (t + 1) * FNB;
+            const bool act = tid >= r1 && tid < n;
+            uint32_t off = Aout.offset(act, tid, t * FNB);
+            const uint32_t step = act ? Aout.ld_bytes : 0u;
+#pragma unroll
+            for(int j = 0; j < FNB; ++j)
             {
-                const int row = e % NMAX, j = e / NMAX;
-                if(row >= r1 && row < n)
-                    sbuf[j * NMAX + row] = Ao[row + int64_t(cb + j) * ldo];
+                Aout.store(cur[j], off, 0);
+                off += step;
+                opaque(off);
             }
-            __syncthreads();
-            for(int e = tid; e < FHC * NMAX; e += NMAX)
-            {
-                const int row = e % NMAX, j = e / NMAX;
-                if(row >= r1 && row < n)
-                    Ao[row + int64_t(cb + j) * ldo] = sbuf[j * NMAX + ssig[t * NMAX + row]];
-            }
-            __syncthreads();
+#pragma unroll
+            for(int j = 0; j < FNB; ++j)
+                cur[j] = nxt[j];
         }
     }
     return zero_at;
