@@ -209,6 +209,87 @@ rocblas_status rocsolver_dgesv_strided_batched(rocblas_handle handle, const rocb
                                                const rocblas_stride strideB, rocblas_int* info,
                                                const rocblas_int batch_count);
 
+/* ------------------------------------------------------------------ GETRI
+ * Inverse from the getrf factors (LU and ipiv) in place, or into C
+ * (out of place); the npvt forms use factors made without pivoting (no ipiv).
+ * info = first zero diagonal entry of U (1-based); the inverse is then
+ * undefined.  The in-place forms take n*n*batch_count elements of handle
+ * workspace (size query supported).  Reference declarations
+ * (rocsolver-functions.h, s / d): getri :9173/:9180, getri_batched
+ * :9245/:9254, getri_strided_batched :9329/:9339, getri_npvt :9402/:9408,
+ * getri_npvt_batched :9463/:9470, getri_npvt_strided_batched :9532/:9540,
+ * getri_outofplace :21380/:21389, getri_outofplace_batched :21462/:21473,
+ * getri_outofplace_strided_batched :21562/:21576, getri_npvt_outofplace
+ * :21653/:21661, getri_npvt_outofplace_batched :21725/:21734,
+ * getri_npvt_outofplace_strided_batched :21813/:21825; implementation
+ * library/src/lapack/roclapack_getri*.{cpp,hpp}. */
+rocblas_status rocsolver_sgetri(rocblas_handle handle, const rocblas_int n, float* A,
+    const rocblas_int lda, rocblas_int* ipiv, rocblas_int* info);
+rocblas_status rocsolver_sgetri_batched(rocblas_handle handle, const rocblas_int n,
+    float* const A[], const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_strided_batched(rocblas_handle handle, const rocblas_int n,
+    float* A, const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,
+    const rocblas_stride strideP, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_npvt(rocblas_handle handle, const rocblas_int n, float* A,
+    const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_sgetri_npvt_batched(rocblas_handle handle, const rocblas_int n,
+    float* const A[], const rocblas_int lda, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_npvt_strided_batched(rocblas_handle handle, const rocblas_int n,
+    float* A, const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_outofplace(rocblas_handle handle, const rocblas_int n, float* A,
+    const rocblas_int lda, rocblas_int* ipiv, float* C, const rocblas_int ldc, rocblas_int* info);
+rocblas_status rocsolver_sgetri_outofplace_batched(rocblas_handle handle, const rocblas_int n,
+    float* const A[], const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP,
+    float* const C[], const rocblas_int ldc, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_outofplace_strided_batched(rocblas_handle handle,
+    const rocblas_int n, float* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* ipiv, const rocblas_stride strideP, float* C, const rocblas_int ldc,
+    const rocblas_stride strideC, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_npvt_outofplace(rocblas_handle handle, const rocblas_int n,
+    float* A, const rocblas_int lda, float* C, const rocblas_int ldc, rocblas_int* info);
+rocblas_status rocsolver_sgetri_npvt_outofplace_batched(rocblas_handle handle, const rocblas_int n,
+    float* const A[], const rocblas_int lda, float* const C[], const rocblas_int ldc,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sgetri_npvt_outofplace_strided_batched(rocblas_handle handle,
+    const rocblas_int n, float* A, const rocblas_int lda, const rocblas_stride strideA, float* C,
+    const rocblas_int ldc, const rocblas_stride strideC, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri(rocblas_handle handle, const rocblas_int n, double* A,
+    const rocblas_int lda, rocblas_int* ipiv, rocblas_int* info);
+rocblas_status rocsolver_dgetri_batched(rocblas_handle handle, const rocblas_int n,
+    double* const A[], const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_strided_batched(rocblas_handle handle, const rocblas_int n,
+    double* A, const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,
+    const rocblas_stride strideP, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_npvt(rocblas_handle handle, const rocblas_int n, double* A,
+    const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_dgetri_npvt_batched(rocblas_handle handle, const rocblas_int n,
+    double* const A[], const rocblas_int lda, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_npvt_strided_batched(rocblas_handle handle, const rocblas_int n,
+    double* A, const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_outofplace(rocblas_handle handle, const rocblas_int n, double* A,
+    const rocblas_int lda, rocblas_int* ipiv, double* C, const rocblas_int ldc, rocblas_int* info);
+rocblas_status rocsolver_dgetri_outofplace_batched(rocblas_handle handle, const rocblas_int n,
+    double* const A[], const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP,
+    double* const C[], const rocblas_int ldc, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_outofplace_strided_batched(rocblas_handle handle,
+    const rocblas_int n, double* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* ipiv, const rocblas_stride strideP, double* C, const rocblas_int ldc,
+    const rocblas_stride strideC, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_npvt_outofplace(rocblas_handle handle, const rocblas_int n,
+    double* A, const rocblas_int lda, double* C, const rocblas_int ldc, rocblas_int* info);
+rocblas_status rocsolver_dgetri_npvt_outofplace_batched(rocblas_handle handle, const rocblas_int n,
+    double* const A[], const rocblas_int lda, double* const C[], const rocblas_int ldc,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dgetri_npvt_outofplace_strided_batched(rocblas_handle handle,
+    const rocblas_int n, double* A, const rocblas_int lda, const rocblas_stride strideA, double* C,
+    const rocblas_int ldc, const rocblas_stride strideC, rocblas_int* info,
+    const rocblas_int batch_count);
+
 /* ------------------------------------------------------------------ LASWP
  * Row interchanges driven by ipiv[k1-1 .. k2-1] (incx = +-1).
  * Reference: rocsolver-functions.h:198/207;

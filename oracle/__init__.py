@@ -55,6 +55,14 @@ def lib() -> ctypes.CDLL:
             getattr(L, f"orc_{pre}getrs").restype = None
             getattr(L, f"orc_{pre}gesv").argtypes = [i, i, p, i, p, p, i]
             getattr(L, f"orc_{pre}gesv").restype = i
+            getattr(L, f"orc_{pre}getri").argtypes = [i, p, i, p]
+            getattr(L, f"orc_{pre}getri").restype = i
+            getattr(L, f"orc_{pre}potf2").argtypes = [c, i, p, i]
+            getattr(L, f"orc_{pre}potf2").restype = i
+            getattr(L, f"orc_{pre}potrs").argtypes = [c, i, i, p, i, p, i]
+            getattr(L, f"orc_{pre}potrs").restype = None
+            getattr(L, f"orc_{pre}posv").argtypes = [c, i, i, p, i, p, i]
+            getattr(L, f"orc_{pre}posv").restype = i
         L.orc_dsgesv.argtypes = [i, i, p, i, p, i, p, i, i, p]
         L.orc_dsgesv.restype = i
         L.orc_dresidual.argtypes = [c, i, i, p, i, p, i, p, i, p, i]
@@ -156,6 +164,57 @@ def gesv(A: np.ndarray, n: int, B: np.ndarray):
     for b in range(bc):
         info[b] = f(n, nrhs, _ptr(A[b]), lda, _ptr(ipiv[b]), _ptr(B[b]), ldb)
     return ipiv[:, :n], info
+
+
+def getri(LU: np.ndarray, n: int, ipiv):
+    """Inverse from the getf2 output of every matrix of a (bc, n, lda) batch, in
+    place; ipiv None for the no-pivoting variants.  Returns info[bc]."""
+    LU = _as_batch(LU)
+    bc, _, lda = LU.shape
+    info = np.zeros(bc, dtype=np.int32)
+    f = getattr(lib(), f"orc_{_prefix(LU.dtype)}getri")
+    if ipiv is not None:
+        ipiv = np.ascontiguousarray(ipiv, dtype=np.int32).reshape(bc, -1)
+    for b in range(bc):
+        info[b] = f(n, _ptr(LU[b]), lda, None if ipiv is None else _ptr(ipiv[b]))
+    return info
+
+
+def potf2(A: np.ndarray, n: int, uplo: str):
+    """Cholesky factorisation (uplo 'L' or 'U') of every matrix of a batch, in
+    place.  Returns info[bc]."""
+    A = _as_batch(A)
+    bc, _, lda = A.shape
+    info = np.zeros(bc, dtype=np.int32)
+    f = getattr(lib(), f"orc_{_prefix(A.dtype)}potf2")
+    u = uplo.upper().encode()
+    for b in range(bc):
+        info[b] = f(u, n, _ptr(A[b]), lda)
+    return info
+
+
+def potrs(A: np.ndarray, n: int, B: np.ndarray, uplo: str) -> None:
+    """Solve with the Cholesky factor of potf2, B (bc, nrhs, ldb) in place."""
+    A, B = _as_batch(A), _as_batch(B)
+    bc, _, lda = A.shape
+    _, nrhs, ldb = B.shape
+    f = getattr(lib(), f"orc_{_prefix(A.dtype)}potrs")
+    u = uplo.upper().encode()
+    for b in range(bc):
+        f(u, n, nrhs, _ptr(A[b]), lda, _ptr(B[b]), ldb)
+
+
+def posv(A: np.ndarray, n: int, B: np.ndarray, uplo: str):
+    """potf2 + potrs per batch entry (B unchanged where info != 0) -> info."""
+    A, B = _as_batch(A), _as_batch(B)
+    bc, _, lda = A.shape
+    _, nrhs, ldb = B.shape
+    info = np.zeros(bc, dtype=np.int32)
+    f = getattr(lib(), f"orc_{_prefix(A.dtype)}posv")
+    u = uplo.upper().encode()
+    for b in range(bc):
+        info[b] = f(u, n, nrhs, _ptr(A[b]), lda, _ptr(B[b]), ldb)
+    return info
 
 
 def dsgesv(A: np.ndarray, n: int, B: np.ndarray, itermax: int = 30):

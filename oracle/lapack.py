@@ -171,6 +171,59 @@ def gesv_one(n: int, nrhs: int, a_ptr: int, lda: int, ipiv_ptr: int, b_ptr: int,
     return inf.value
 
 
+def getri(LU: np.ndarray, n: int, ipiv: np.ndarray):
+    """xGETRI on every matrix of a (bc, n, lda) batch of getrf factors, in place -> info."""
+    if LU.ndim == 2:
+        LU = LU[None]
+    bc, _, lda = LU.shape
+    pre = "d" if LU.dtype == np.float64 else "s"
+    f = _fn(pre + "getri_")
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32).reshape(bc, -1)
+    lwork = max(1, 64 * n)
+    work = np.zeros(lwork, dtype=LU.dtype)
+    info = np.zeros(bc, dtype=np.int32)
+    inf = _I()
+    for b in range(bc):
+        f(_byref_int(n), _P(LU[b].ctypes.data), _byref_int(lda), _P(ipiv[b].ctypes.data),
+          _P(work.ctypes.data), _byref_int(lwork), ctypes.byref(inf))
+        info[b] = inf.value
+    return info
+
+
+def potrf(A: np.ndarray, n: int, uplo: str):
+    """xPOTRF per batch entry, in place -> info."""
+    if A.ndim == 2:
+        A = A[None]
+    bc, _, lda = A.shape
+    pre = "d" if A.dtype == np.float64 else "s"
+    f = _fn(pre + "potrf_")
+    u = ctypes.c_char(uplo.upper().encode()[:1])
+    info = np.zeros(bc, dtype=np.int32)
+    inf = _I()
+    for b in range(bc):
+        f(ctypes.byref(u), _byref_int(n), _P(A[b].ctypes.data), _byref_int(lda), ctypes.byref(inf),
+          ctypes.c_size_t(1))
+        info[b] = inf.value
+    return info
+
+
+def potrs(A: np.ndarray, n: int, B: np.ndarray, uplo: str) -> None:
+    """xPOTRS per batch entry, B (bc, nrhs, ldb) in place."""
+    if A.ndim == 2:
+        A = A[None]
+    if B.ndim == 2:
+        B = B[None]
+    bc, _, lda = A.shape
+    _, nrhs, ldb = B.shape
+    pre = "d" if A.dtype == np.float64 else "s"
+    f = _fn(pre + "potrs_")
+    u = ctypes.c_char(uplo.upper().encode()[:1])
+    inf = _I()
+    for b in range(bc):
+        f(ctypes.byref(u), _byref_int(n), _byref_int(nrhs), _P(A[b].ctypes.data), _byref_int(lda),
+          _P(B[b].ctypes.data), _byref_int(ldb), ctypes.byref(inf), ctypes.c_size_t(1))
+
+
 def dsgesv(A: np.ndarray, n: int, B: np.ndarray):
     """LAPACK DSGESV per batch entry -> (X, iter, info); A is overwritten only on fallback."""
     if A.ndim == 2:

@@ -21,6 +21,20 @@
  *          (column-oriented substitution; non-unit diagonal by true division)
  *  gesv    library/src/lapack/roclapack_gesv.hpp:145-175 (B unchanged where
  *          info != 0)
+ *  getri   library/src/lapack/roclapack_getri.hpp:265-378 (inv(U) in place by
+ *          trtri, upper non-unit, whose singularity check leaves info = first
+ *          zero diagonal; then inv(A) from inv(A)*L = inv(U) column by column
+ *          from the right; then the column interchanges of ipiv in reverse,
+ *          getri_pivot :56-70), in LAPACK dgetri's unblocked form (dtrti2 +
+ *          dgemv sweep)
+ *  potf2   library/src/lapack/roclapack_potf2.hpp:10-37,160-250 (left-looking:
+ *          t = a_jj - dot; t <= 0 -> info = first j+1, a_jj = t, scale 0, and
+ *          the factorisation goes on; else a_jj = sqrt(t), then the rest of
+ *          column j (lower) / row j (upper) minus a gemv, times 1/a_jj)
+ *  potrs   library/src/lapack/roclapack_potrs.hpp:121-148 (L then L^T, or U^T
+ *          then U, non-unit triangular solves)
+ *  posv    library/src/lapack/roclapack_posv.hpp:140-170 (potrf, potrs, B
+ *          unchanged where info != 0)
  *  dsgesv  no rocSOLVER counterpart; LAPACK 3.x dsgesv.f semantics (range
  *          check -> ITER=-2, FP32 LU zero pivot -> ITER=-3, ITERMAX steps,
  *          converged when ||r||_inf <= ||x||_inf ||A||_inf sqrt(n) eps,
@@ -221,6 +235,132 @@ DEF_GETRS(float, s, fmaf)
 
 DEF_GESV(double, d)
 DEF_GESV(float, s)
+
+/* ------------------------------------------------------------------ getri */
+#define DEF_GETRI(T, P, FMA)                                                            \
+    int orc_##P##getri(int n, T* A, int lda, const int* ipiv)                         \
+    {                                                                                   \
+        for(int j = 0; j < n; ++j)                                                      \
+            if(A[IDX(j, j, lda)] == (T)0)                                               \
+                return j + 1;                                                           \
+        /* inv(U): column j of the inverse from the inverted leading block */           \
+        for(int j = 0; j < n; ++j)                                                      \
+        {                                                                               \
+            A[IDX(j, j, lda)] = (T)1 / A[IDX(j, j, lda)];                               \
+            T ajj = -A[IDX(j, j, lda)];                                                 \
+            for(int jj = 0; jj < j; ++jj)                                               \
+            {                                                                           \
+                T t = A[IDX(jj, j, lda)];                                               \
+                for(int i = 0; i < jj; ++i)                                             \
+                    A[IDX(i, j, lda)] = FMA(t, A[IDX(i, jj, lda)], A[IDX(i, j, lda)]);  \
+                A[IDX(jj, j, lda)] = t * A[IDX(jj, jj, lda)];                           \
+            }                                                                           \
+            for(int i = 0; i < j; ++i)                                                  \
+                A[IDX(i, j, lda)] *= ajj;                                               \
+        }                                                                               \
+        /* inv(A) * L = inv(U), columns from the right */                               \
+        T* work = (T*)malloc(sizeof(T) * (size_t)(n > 0 ? n : 1));                      \
+        for(int j = n - 1; j >= 0; --j)                                                 \
+        {                                                                               \
+            for(int i = j + 1; i < n; ++i)                                              \
+            {                                                                           \
+                work[i] = A[IDX(i, j, lda)];                                            \
+                A[IDX(i, j, lda)] = (T)0;                                               \
+            }                                                                           \
+            for(int jj = j + 1; jj < n; ++jj)                                           \
+                for(int i = 0; i < n; ++i)                                              \
+                    A[IDX(i, j, lda)] = FMA(-A[IDX(i, jj, lda)], work[jj], A[IDX(i, j, lda)]); \
+        }                                                                               \
+        free(work);                                                                     \
+        if(ipiv)                                                                        \
+            for(int j = n - 2; j >= 0; --j)                                             \
+            {                                                                           \
+                int jp = ipiv[j] - 1;                                                   \
+                if(jp != j)                                                             \
+                    for(int i = 0; i < n; ++i)                                          \
+                    {                                                                   \
+                        T t = A[IDX(i, j, lda)];                                        \
+                        A[IDX(i, j, lda)] = A[IDX(i, jp, lda)];                         \
+                        A[IDX(i, jp, lda)] = t;                                         \
+                    }                                                                   \
+            }                                                                           \
+        return 0;                                                                       \
+    }
+
+DEF_GETRI(double, d, fma)
+DEF_GETRI(float, s, fmaf)
+
+/* ------------------------------------------------------------ potf2 etc. */
+/* element (i, j) of the lower factor L: A(i, j) (lower) or A(j, i) (upper,
+ * where the factor is U = L^T) */
+#define LEL(i, j) (*(up ? &A[IDX(j, i, lda)] : &A[IDX(i, j, lda)]))
+#define DEF_POTF2(T, P, SQRT, FMA)                                                      \
+    int orc_##P##potf2(char uplo, int n, T* A, int lda)                                \
+    {                                                                                   \
+        int up = (uplo == 'U' || uplo == 'u');                                          \
+        int info = 0;                                                                   \
+        for(int j = 0; j < n; ++j)                                                      \
+        {                                                                               \
+            T dot = (T)0;                                                               \
+            for(int k = 0; k < j; ++k)                                                  \
+                dot = FMA(LEL(j, k), LEL(j, k), dot);                                   \
+            T t = LEL(j, j) - dot;                                                      \
+            T r;                                                                        \
+            if(t <= (T)0)                                                               \
+            {                                                                           \
+                if(info == 0)                                                           \
+                    info = j + 1;                                                       \
+                LEL(j, j) = t;                                                          \
+                r = (T)0;                                                               \
+            }                                                                           \
+            else                                                                        \
+            {                                                                           \
+                LEL(j, j) = SQRT(t);                                                    \
+                r = (T)1 / LEL(j, j);                                                   \
+            }                                                                           \
+            for(int i = j + 1; i < n; ++i)                                              \
+            {                                                                           \
+                T v = LEL(i, j);                                                        \
+                for(int k = 0; k < j; ++k)                                              \
+                    v = FMA(-LEL(i, k), LEL(j, k), v);                                  \
+                LEL(i, j) = v * r;                                                      \
+            }                                                                           \
+        }                                                                               \
+        return info;                                                                    \
+    }                                                                                   \
+    void orc_##P##potrs(char uplo, int n, int nrhs, const T* A, int lda, T* B, int ldb) \
+    {                                                                                   \
+        int up = (uplo == 'U' || uplo == 'u');                                          \
+        for(int c = 0; c < nrhs; ++c)                                                   \
+        {                                                                               \
+            T* b = B + IDX(0, c, ldb);                                                  \
+            /* L y = b (column oriented) */                                             \
+            for(int k = 0; k < n; ++k)                                                  \
+            {                                                                           \
+                b[k] = b[k] / LEL(k, k);                                                \
+                for(int i = k + 1; i < n; ++i)                                          \
+                    b[i] = FMA(-LEL(i, k), b[k], b[i]);                                 \
+            }                                                                           \
+            /* L^T x = y */                                                             \
+            for(int k = n - 1; k >= 0; --k)                                             \
+            {                                                                           \
+                b[k] = b[k] / LEL(k, k);                                                \
+                for(int i = 0; i < k; ++i)                                              \
+                    b[i] = FMA(-LEL(k, i), b[k], b[i]);                                 \
+            }                                                                           \
+        }                                                                               \
+    }                                                                                   \
+    int orc_##P##posv(char uplo, int n, int nrhs, T* A, int lda, T* B, int ldb)        \
+    {                                                                                   \
+        int info = orc_##P##potf2(uplo, n, A, lda);                                     \
+        if(info == 0)                                                                   \
+            orc_##P##potrs(uplo, n, nrhs, A, lda, B, ldb);                              \
+        return info;                                                                    \
+    }
+
+DEF_POTF2(double, d, sqrt, fma)
+DEF_POTF2(float, s, sqrtf, fmaf)
+#undef LEL
 
 /* ------------------------------------------------------------- residuals */
 void orc_dresidual(char trans, int n, int nrhs, const double* A, int lda, const double* X,

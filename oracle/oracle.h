@@ -1,6 +1,6 @@
 /* oracle.h — CPU restatement of the reference LU path (getf2/getrf, laswp,
- * getrs, gesv, and the mixed-precision refinement of LAPACK xGESV's
- * dsgesv).
+ * getrs, gesv, getri, the Cholesky potf2/potrs/posv, and the
+ * mixed-precision refinement of LAPACK dsgesv).
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this code: only tests/, __graft_entry__.smoke() and bench.py's
@@ -46,6 +46,25 @@ void orc_sgetrs(char trans, int n, int nrhs, const float* A, int lda, const int*
 /* getf2 + getrs; B is left unchanged when info != 0.  Returns info. */
 int orc_dgesv(int n, int nrhs, double* A, int lda, int* ipiv, double* B, int ldb);
 int orc_sgesv(int n, int nrhs, float* A, int lda, int* ipiv, float* B, int ldb);
+
+/* Inverse from the getf2 output (LU, ipiv), in place (LAPACK dgetri, unblocked).
+ * ipiv NULL: no column interchanges (the no-pivoting variants).  Returns info
+ * (first zero diagonal of U, 1-based; A untouched then) or 0. */
+int orc_dgetri(int n, double* A, int lda, const int* ipiv);
+int orc_sgetri(int n, float* A, int lda, const int* ipiv);
+
+/* Cholesky factorisation, uplo 'L' (A = L L^T) or 'U' (A = U^T U), of the
+ * referenced triangle in place.  A non-positive pivot t sets info (first,
+ * 1-based), stores t on the diagonal, scales its column (row) by 0 and the
+ * factorisation goes on, as the reference does.  Returns info. */
+int orc_dpotf2(char uplo, int n, double* A, int lda);
+int orc_spotf2(char uplo, int n, float* A, int lda);
+/* Solve A X = B with the factor of potf2. */
+void orc_dpotrs(char uplo, int n, int nrhs, const double* A, int lda, double* B, int ldb);
+void orc_spotrs(char uplo, int n, int nrhs, const float* A, int lda, float* B, int ldb);
+/* potf2 + potrs; B unchanged when info != 0.  Returns info. */
+int orc_dposv(char uplo, int n, int nrhs, double* A, int lda, double* B, int ldb);
+int orc_sposv(char uplo, int n, int nrhs, float* A, int lda, float* B, int ldb);
 
 /* Mixed-precision solve (LAPACK dsgesv semantics): FP32 LU of A, FP32
  * correction solves, FP64 residual and update, stop when every column

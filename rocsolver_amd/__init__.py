@@ -102,6 +102,18 @@ SIGNATURES = {
     "gesv_batched": [_P, _I, _I, _P, _I, _P, _S, _P, _I, _P, _I],
     "gesv_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S, _P, _I],
     "laswp": [_P, _I, _P, _I, _I, _I, _P, _I],
+    "getri": [_P, _I, _P, _I, _P, _P],
+    "getri_batched": [_P, _I, _P, _I, _P, _S, _P, _I],
+    "getri_strided_batched": [_P, _I, _P, _I, _S, _P, _S, _P, _I],
+    "getri_npvt": [_P, _I, _P, _I, _P],
+    "getri_npvt_batched": [_P, _I, _P, _I, _P, _I],
+    "getri_npvt_strided_batched": [_P, _I, _P, _I, _S, _P, _I],
+    "getri_outofplace": [_P, _I, _P, _I, _P, _P, _I, _P],
+    "getri_outofplace_batched": [_P, _I, _P, _I, _P, _S, _P, _I, _P, _I],
+    "getri_outofplace_strided_batched": [_P, _I, _P, _I, _S, _P, _S, _P, _I, _S, _P, _I],
+    "getri_npvt_outofplace": [_P, _I, _P, _I, _P, _I, _P],
+    "getri_npvt_outofplace_batched": [_P, _I, _P, _I, _P, _I, _P, _I],
+    "getri_npvt_outofplace_strided_batched": [_P, _I, _P, _I, _S, _P, _I, _S, _P, _I],
 }
 
 
@@ -262,6 +274,39 @@ def getrs(handle: Handle, LU, ipiv, B, trans: str = "N", batched: bool = False):
     else:
         check(f"rocsolver_{p}getrs_strided_batched", handle.value, op, n, nrhs, LU.data_ptr(),
               lda, n * lda, ipiv.data_ptr(), sp, B.data_ptr(), ldb, nrhs * ldb, bc)
+
+
+def getri(handle: Handle, LU, ipiv=None, C=None, form: str = "strided"):
+    """Inverse from getrf factors LU (bc, n, lda): in place, or into C (bc, n,
+    ldc) when given; ipiv None selects the no-pivoting (npvt) entry points.
+    form: "strided", "batched" (pointer arrays) or "plain" (bc must be 1).
+    Returns info (bc,) int32 on the device."""
+    import torch
+    bc, n, lda = LU.shape
+    p = _prefix(LU)
+    info = torch.zeros(bc, dtype=torch.int32, device=LU.device)
+    npvt = ipiv is None
+    name = f"rocsolver_{p}getri" + ("_npvt" if npvt else "") + ("_outofplace" if C is not None else "")
+    piv = [] if npvt else [ipiv.data_ptr()]
+    if form == "plain":
+        assert bc == 1
+        args = [LU.data_ptr(), lda] + piv
+        if C is not None:
+            args += [C.data_ptr(), C.shape[2]]
+        check(name, handle.value, n, *args, info.data_ptr())
+    elif form == "batched":
+        pa = pointer_array(LU)
+        args = [pa.data_ptr(), lda] + ([] if npvt else [ipiv.data_ptr(), ipiv.stride(0)])
+        if C is not None:
+            pc = pointer_array(C)
+            args += [pc.data_ptr(), C.shape[2]]
+        check(name + "_batched", handle.value, n, *args, info.data_ptr(), bc)
+    else:
+        args = [LU.data_ptr(), lda, n * lda] + ([] if npvt else [ipiv.data_ptr(), ipiv.stride(0)])
+        if C is not None:
+            args += [C.data_ptr(), C.shape[2], C.stride(0)]
+        check(name + "_strided_batched", handle.value, n, *args, info.data_ptr(), bc)
+    return info
 
 
 def gesv(handle: Handle, A, B, batched: bool = False):

@@ -157,6 +157,44 @@ rocblas_status laswp_impl(rocblas_handle handle, int n, T* A, int lda, int k1, i
     return laswp<T>(stream, n, strided(A, 0), lda, k1, k2, ipiv, 0, incx, 1, nullptr);
 }
 
+// getri / getri_outofplace (and the no-pivoting forms): argument checks of
+// roclapack_getri.hpp:216-242 and roclapack_getri_outofplace.hpp:40-67
+// (sizes, then pointers unless querying), quick returns of :280-295.  The
+// in-place form takes n*n*bc elements of workspace for a copy of the factors.
+template <typename T>
+rocblas_status getri_impl(rocblas_handle handle, int n, Batch<T> A, bool hasA, int lda,
+                          int* ipiv, int64_t strideP, bool outofplace, Batch<T> C, bool hasC,
+                          int ldc, int* info, int bc, bool pivot)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    if(n < 0 || lda < n || (outofplace && ldc < n) || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query
+       && ((n && !hasA) || (outofplace && n && !hasC) || (n && pivot && !ipiv) || (bc && !info)))
+        return rocblas_status_invalid_pointer;
+    const size_t ws = (!outofplace && n > 0 && bc > 0) ? sizeof(T) * size_t(n) * n * size_t(bc) : 0;
+    if(query)
+        return ws ? rocblas_set_optimal_device_memory_size(handle, ws) : report_no_workspace(handle);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(n == 0)
+        return reset_info(stream, info, bc);
+    if(outofplace)
+        return getri_solve<T>(stream, n, A, lda, ipiv, strideP, C, ldc, info, bc, pivot);
+    rocblas_device_malloc mem(handle, ws);
+    if(!mem)
+        return rocblas_status_memory_error;
+    Batch<T> W = strided(static_cast<T*>(static_cast<void*>(mem)), int64_t(n) * n);
+    rocblas_status st = copy_square<T>(stream, n, A, lda, W, n, bc);
+    if(st != rocblas_status_success)
+        return st;
+    return getri_solve<T>(stream, n, W, n, ipiv, strideP, A, lda, info, bc, pivot);
+}
+
 constexpr int DSGESV_ITERMAX = 30; // LAPACK dsgesv ITERMAX
 
 rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, const double* A,
@@ -523,6 +561,105 @@ rocblas_status rocsolver_amd_dsgetrf_strided_batched(
     }
 RSA_ADJOINT(s, float)
 RSA_ADJOINT(d, double)
+
+#define RSA_GETRI(P, T)                                                                         \
+    rocblas_status rocsolver_##P##getri(rocblas_handle handle, const rocblas_int n, T* A,       \
+                                        const rocblas_int lda, rocblas_int* ipiv,               \
+                                        rocblas_int* info)                                      \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, ipiv, 0, false,       \
+                             strided<T>(nullptr, 0), false, 0, info, 1, true);                  \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_batched(rocblas_handle handle, const rocblas_int n,     \
+                                                T* const A[], const rocblas_int lda,            \
+                                                rocblas_int* ipiv, const rocblas_stride strideP, \
+                                                rocblas_int* info, const rocblas_int batch_count) \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, ipiv, strideP, false,    \
+                             strided<T>(nullptr, 0), false, 0, info, batch_count, true);        \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_int n, T* A, const rocblas_int lda,                \
+        const rocblas_stride strideA, rocblas_int* ipiv, const rocblas_stride strideP,          \
+        rocblas_int* info, const rocblas_int batch_count)                                       \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, ipiv, strideP,   \
+                             false, strided<T>(nullptr, 0), false, 0, info, batch_count, true); \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt(rocblas_handle handle, const rocblas_int n, T* A,  \
+                                             const rocblas_int lda, rocblas_int* info)          \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, nullptr, 0, false,    \
+                             strided<T>(nullptr, 0), false, 0, info, 1, false);                 \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt_batched(rocblas_handle handle, const rocblas_int n, \
+                                                     T* const A[], const rocblas_int lda,       \
+                                                     rocblas_int* info,                         \
+                                                     const rocblas_int batch_count)             \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, nullptr, 0, false,       \
+                             strided<T>(nullptr, 0), false, 0, info, batch_count, false);       \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt_strided_batched(                                   \
+        rocblas_handle handle, const rocblas_int n, T* A, const rocblas_int lda,                \
+        const rocblas_stride strideA, rocblas_int* info, const rocblas_int batch_count)         \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,      \
+                             false, strided<T>(nullptr, 0), false, 0, info, batch_count, false); \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_outofplace(rocblas_handle handle, const rocblas_int n,  \
+                                                   T* A, const rocblas_int lda,                 \
+                                                   rocblas_int* ipiv, T* C,                     \
+                                                   const rocblas_int ldc, rocblas_int* info)    \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, ipiv, 0, true,       \
+                             strided(C, 0), C != nullptr, ldc, info, 1, true);                  \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_outofplace_batched(                                     \
+        rocblas_handle handle, const rocblas_int n, T* const A[], const rocblas_int lda,        \
+        rocblas_int* ipiv, const rocblas_stride strideP, T* const C[], const rocblas_int ldc,   \
+        rocblas_int* info, const rocblas_int batch_count)                                       \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, ipiv, strideP, true,     \
+                             arrayed(C), C != nullptr, ldc, info, batch_count, true);           \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_outofplace_strided_batched(                             \
+        rocblas_handle handle, const rocblas_int n, T* A, const rocblas_int lda,                \
+        const rocblas_stride strideA, rocblas_int* ipiv, const rocblas_stride strideP, T* C,    \
+        const rocblas_int ldc, const rocblas_stride strideC, rocblas_int* info,                 \
+        const rocblas_int batch_count)                                                          \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, ipiv, strideP,   \
+                             true, strided(C, strideC), C != nullptr, ldc, info, batch_count,   \
+                             true);                                                             \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt_outofplace(rocblas_handle handle,                  \
+                                                        const rocblas_int n, T* A,              \
+                                                        const rocblas_int lda, T* C,            \
+                                                        const rocblas_int ldc,                  \
+                                                        rocblas_int* info)                      \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, nullptr, 0, true,     \
+                             strided(C, 0), C != nullptr, ldc, info, 1, false);                 \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt_outofplace_batched(                                \
+        rocblas_handle handle, const rocblas_int n, T* const A[], const rocblas_int lda,        \
+        T* const C[], const rocblas_int ldc, rocblas_int* info, const rocblas_int batch_count)  \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, nullptr, 0, true,        \
+                             arrayed(C), C != nullptr, ldc, info, batch_count, false);          \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##getri_npvt_outofplace_strided_batched(                        \
+        rocblas_handle handle, const rocblas_int n, T* A, const rocblas_int lda,                \
+        const rocblas_stride strideA, T* C, const rocblas_int ldc, const rocblas_stride strideC, \
+        rocblas_int* info, const rocblas_int batch_count)                                       \
+    {                                                                                           \
+        return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,      \
+                             true, strided(C, strideC), C != nullptr, ldc, info, batch_count,   \
+                             false);                                                            \
+    }
+RSA_GETRI(s, float)
+RSA_GETRI(d, double)
 
 #define RSA_VBATCHED(P, T)                                                                      \
     rocblas_status rocsolver_amd_##P##getrf_vbatched(                                           \

@@ -126,6 +126,17 @@ rocblas_status vbatched(hipStream_t stream, int op, bool trans, const int* n, co
                         T* const* A, const int* lda, int* const* ipiv, T* const* B,
                         const int* ldb, int* info, int bc, void* work);
 
+// ------------------------------------------------------------------- getri
+// X = inv(A) from its LU factors (LU, ipiv; pivot = false: no interchanges):
+// info = first zero diagonal entry of U, X = I, then A X = I solved in place
+// in X.  X must not alias LU.
+template <typename T>
+rocblas_status getri_solve(hipStream_t stream, int n, Batch<T> LU, int ldlu, const int* ipiv,
+                           int64_t strideP, Batch<T> X, int ldx, int* info, int bc, bool pivot);
+template <typename T>
+rocblas_status copy_square(hipStream_t stream, int n, Batch<T> S, int lds, Batch<T> D, int ldd,
+                           int bc);
+
 // --------------------------------------------------------------- utilities
 rocblas_status reset_info(hipStream_t stream, int* info, int bc);
 

@@ -99,6 +99,53 @@ def reference_getrs(n: int, nrhs: int, lda: int, ldb: int, bc: int, dtype=np.flo
     return A, B
 
 
+def _reference_zero_diagonal_positions(n: int, b: int):
+    """Diagonal positions the reference tests zero in their singular batch
+    entries (n/4 + b, n/2 + b, n - 1 + b, each mod n)."""
+    return [j0 - (j0 // n) * n for j0 in (n // 4 + b, n // 2 + b, n - 1 + b)]
+
+
+def reference_getri(n: int, lda: int, bc: int, dtype=np.float64) -> np.ndarray:
+    """Input of the reference getri tests before their host LU
+    (testing_getri.hpp:117-140): diagonal v/10 + 10, off-diagonal (v - 4)/10,
+    rows reversed.  The singular entries are made after the LU by zeroing
+    diagonal entries of U (reference_getri_singular)."""
+    A = _reference_fill(bc, n, lda, dtype)
+    for b in range(bc):
+        M = A[b]  # M[j, i] = A(i, j)
+        for j in range(n):
+            for i in range(n):
+                M[j, i] = M[j, i] / 10.0 + 10 if i == j else (M[j, i] - 4) / 10.0
+        rows = M[:n, :n].copy()
+        M[:n, :n] = rows[:, ::-1]
+    return A
+
+
+def reference_getri_singular(LU: np.ndarray, n: int) -> None:
+    """Zero the diagonal entries of U that the reference getri tests zero in
+    batch entries bc/4, bc/2 and bc-1 (testing_getri.hpp:142-157), in place."""
+    bc = LU.shape[0]
+    for b in range(bc):
+        if b == bc // 4 or b == bc // 2 or b == bc - 1:
+            for i in _reference_zero_diagonal_positions(n, b):
+                LU[b, i, i] = 0
+
+
+def reference_potrf(n: int, lda: int, bc: int, singular: bool = False,
+                    dtype=np.float64) -> np.ndarray:
+    """Input of the reference potf2/potrf tests (testing_potf2_potrf.hpp:116-140):
+    the clients' integers with each diagonal entry a replaced by a*a*400; in
+    the singular entries (bc/4, bc/2, bc-1) three diagonal entries are 0."""
+    A = _reference_fill(bc, n, lda, dtype)
+    for b in range(bc):
+        for i in range(n):
+            A[b, i, i] = A[b, i, i] * A[b, i, i] * 400
+        if singular and (b == bc // 4 or b == bc // 2 or b == bc - 1):
+            for i in _reference_zero_diagonal_positions(n, b):
+                A[b, i, i] = 0
+    return A
+
+
 # ----------------------------------------------------------------- configs
 def rng(config_index: int, seed: int = BASE_SEED) -> np.random.Generator:
     return np.random.default_rng(seed + config_index)

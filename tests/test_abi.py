@@ -144,3 +144,19 @@ def test_dxgesv_invalid_factor():
     for f in (-1, 3):
         assert ra.call("rocsolver_amd_dxgesv_strided_batched", DUMMY, f, 4, 1, DUMMY, 4, 16,
                        DUMMY, 4, 4, DUMMY, 4, 4, DUMMY, DUMMY, None, 1) == 11
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_getri_argument_checks(p):
+    """getri argCheck (roclapack_getri.hpp:216-242, roclapack_getri_outofplace.hpp:
+    40-67): null handle first, then n < 0 || lda < n (|| ldc < n) || bc < 0."""
+    f = f"rocsolver_{p}getri_strided_batched"
+    assert ra.call(f, None, 4, DUMMY, 4, 16, DUMMY, 4, DUMMY, 1) == 1
+    for n, lda, bc in [(-1, 4, 1), (4, 3, 1), (4, 4, -1)]:
+        assert ra.call(f, DUMMY, n, DUMMY, lda, 16, DUMMY, 4, DUMMY, bc) == 4
+        assert ra.call(f"rocsolver_{p}getri_npvt_strided_batched", DUMMY, n, DUMMY, lda, 16,
+                       DUMMY, bc) == 4
+    g = f"rocsolver_{p}getri_outofplace_strided_batched"
+    assert ra.call(g, DUMMY, 4, DUMMY, 4, 16, DUMMY, 4, DUMMY, 3, 16, DUMMY, 1) == 4  # ldc < n
+    assert ra.call(f"rocsolver_{p}getri_npvt_outofplace_batched", DUMMY, 4, DUMMY, 4, DUMMY, 2,
+                   DUMMY, 1) == 4

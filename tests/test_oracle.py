@@ -24,6 +24,7 @@ needs_lapack = pytest.mark.skipif(not lapack.available(), reason="no host LAPACK
 
 
 def rel_fro(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
     return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
 
 
@@ -428,3 +429,187 @@ def test_lu_residual_reports_a_known_perturbation():
     ip2[0] = 1 if ipiv[0] != 1 else 2
     r3 = lib.orc_dlu_residual(n, n, A0.ctypes.data, n, LU.ctypes.data, n, ip2.ctypes.data)
     assert r3 > 1.0
+
+
+# ------------------------------------------------------------------ getri
+# reference check-in sizes (n, lda) of getri_gtest.cpp:28-36 plus its first large size
+GETRI_SIZES = [(1, 1), (20, 20), (32, 32), (50, 50), (100, 150), (192, 192)]
+
+
+@needs_lapack
+@pytest.mark.parametrize("n,lda", GETRI_SIZES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_getri_vs_lapack_reference_matrices(n, lda, dtype):
+    """Inverse of the reference getri test matrices (host LU, then getri) equals
+    LAPACK xGETRI within the reference threshold n*eps (testing_getri.hpp:224-233,465);
+    the zeroed diagonal entries of U give LAPACK's info exactly."""
+    A = wl.reference_getri(n, lda, 4, dtype=dtype)
+    LU = A.copy()
+    ipiv, info = lapack.getrf(LU, n)
+    assert (info == 0).all()
+    wl.reference_getri_singular(LU, n)
+    X = LU.copy()
+    Y = LU.copy()
+    info_o = oracle.getri(X, n, ipiv)
+    info_l = lapack.getri(Y, n, ipiv)
+    np.testing.assert_array_equal(info_o, info_l)
+    for b in range(4):
+        if info_o[b] == 0:
+            assert rel_fro(X[b, :n, :n], Y[b, :n, :n]) <= n * EPS[dtype]
+        else:
+            # LAPACK leaves A untouched when U is singular; so does the oracle
+            np.testing.assert_array_equal(X[b], LU[b])
+
+
+def test_getri_closed_forms():
+    """Permutation matrices invert exactly to their transposes; an integer
+    unimodular matrix inverts exactly; the first zero on the diagonal of U is info."""
+    g = np.random.default_rng(8)
+    n = 9
+    P = np.eye(n)[g.permutation(n)]
+    A = np.ascontiguousarray(P.T)[None]  # column-major storage of P
+    LU = A.copy()
+    ipiv, info = oracle.getf2(LU, n)
+    X = LU.copy()
+    assert oracle.getri(X, n, ipiv)[0] == 0
+    np.testing.assert_array_equal(X[0].T, P.T)  # inverse of P is P^T (exact)
+    # [[2, 1], [1, 1]]^-1 = [[1, -1], [-1, 2]]: LU with pivot 2, multiplier 1/2 (exact)
+    A = np.array([[[2.0, 1.0], [1.0, 1.0]]])  # column-major: columns (2,1) and (1,1)
+    LU = A.copy()
+    ipiv, _ = oracle.getf2(LU, 2)
+    X = LU.copy()
+    oracle.getri(X, 2, ipiv)
+    np.testing.assert_array_equal(X[0].T, np.array([[1.0, -1.0], [-1.0, 2.0]]))
+    # singular U: info names the first zero diagonal entry, A stays as given
+    U = np.triu(g.uniform(1, 2, (6, 6)))
+    U[2, 2] = U[4, 4] = 0.0
+    A = np.ascontiguousarray(U.T)[None]
+    X = A.copy()
+    assert oracle.getri(X, 6, np.arange(1, 7, dtype=np.int32)[None])[0] == 3
+    np.testing.assert_array_equal(X, A)
+
+
+def test_getri_inverse_property_and_npvt():
+    """A X = I for random well-conditioned matrices (a dropped interchange or a
+    transposed sweep fails this); without pivoting on diagonally dominant
+    matrices the npvt form (ipiv None) equals the pivoting one with identity ipiv."""
+    g = np.random.default_rng(9)
+    n = 40
+    M = g.standard_normal((n, n)) + 2 * np.sqrt(n) * np.eye(n)
+    A = np.ascontiguousarray(M.T)[None]
+    LU = A.copy()
+    ipiv, _ = oracle.getf2(LU, n)
+    X = LU.copy()
+    oracle.getri(X, n, ipiv)
+    assert np.abs(M @ X[0].T - np.eye(n)).max() < 100 * n * EPS[np.float64]
+    Dd = g.standard_normal((n, n)) + 4 * n * np.eye(n)
+    A = np.ascontiguousarray(Dd.T)[None]
+    L1 = A.copy()
+    assert oracle.getf2_npvt(L1, n)[0] == 0
+    L2 = A.copy()
+    ip2, _ = oracle.getf2(L2, n)
+    np.testing.assert_array_equal(ip2[0], np.arange(1, n + 1))
+    X1, X2 = L1.copy(), L2.copy()
+    oracle.getri(X1, n, None)
+    oracle.getri(X2, n, ip2)
+    np.testing.assert_array_equal(X1, X2)
+
+
+# ---------------------------------------------------------- potf2 / potrs
+POTRF_SIZES = [(1, 1), (10, 10), (20, 30), (50, 50), (70, 100)]
+
+
+def _lower_of(A, n, uplo):
+    """The factor as a lower-triangular matrix L (A = L L^T) from column-major storage."""
+    M = A[:n, :n].T  # the stored matrix
+    return np.tril(M) if uplo == "L" else np.tril(M.T)
+
+
+@needs_lapack
+@pytest.mark.parametrize("n,lda", POTRF_SIZES)
+@pytest.mark.parametrize("uplo", ["L", "U"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_potf2_vs_lapack_reference_matrices(n, lda, uplo, dtype):
+    """Factor of the reference potrf test matrices equals LAPACK xPOTRF within
+    n*eps (testing_potf2_potrf.hpp:181-206); non-positive-definite entries give
+    LAPACK's info, and the leading block before the failed pivot agrees."""
+    A = wl.reference_potrf(n, lda, 4, singular=True, dtype=dtype)
+    X, Y = A.copy(), A.copy()
+    info_o = oracle.potf2(X, n, uplo)
+    info_l = lapack.potrf(Y, n, uplo)
+    np.testing.assert_array_equal(info_o, info_l)
+    for b in range(4):
+        nn = n if info_o[b] == 0 else int(info_o[b])
+        Lo = _lower_of(X[b], n, uplo)[:nn, :nn]
+        Ll = _lower_of(Y[b], n, uplo)[:nn, :nn]
+        assert rel_fro(Lo, Ll) <= n * EPS[dtype]
+
+
+def test_potf2_closed_forms_and_untouched_triangle():
+    """L L^T of a small-integer L whose diagonal holds exact squares refactors
+    exactly; a diagonal matrix gives its square roots; the other triangle is
+    never read or written."""
+    L = np.array([[2.0, 0, 0, 0], [1.0, 3.0, 0, 0], [-2.0, 1.0, 1.0, 0], [0.0, 2.0, -1.0, 4.0]])
+    S = L @ L.T
+    for uplo in "LU":
+        A = np.ascontiguousarray(S.T)[None].copy()
+        other = np.triu(A[0].T, 1) if uplo == "L" else np.tril(A[0].T, -1)
+        A[0] = A[0] + 0  # copy
+        # poison the unreferenced triangle: it must come back unchanged
+        M = A[0].T.copy()
+        mask = np.triu(np.ones((4, 4), bool), 1) if uplo == "L" else np.tril(np.ones((4, 4), bool), -1)
+        M[mask] = 12345.0
+        A[0] = M.T
+        assert oracle.potf2(A, 4, uplo)[0] == 0
+        np.testing.assert_array_equal(_lower_of(A[0], 4, uplo), L)
+        np.testing.assert_array_equal(A[0].T[mask], 12345.0)
+    D = np.diag([4.0, 9.0, 2.25, 16.0])
+    A = np.ascontiguousarray(D)[None]
+    assert oracle.potf2(A, 4, "L")[0] == 0
+    np.testing.assert_array_equal(np.diag(A[0]), [2.0, 3.0, 1.5, 4.0])
+
+
+def test_potf2_non_positive_pivot_continues():
+    """A non-positive pivot t sets info (first only), stores t, and the column
+    below it is scaled by 0 (the reference's sqrtDiagOnward); later pivots go on."""
+    S = np.diag([4.0, -1.0, 9.0, 0.0, 16.0])
+    S[2, 1] = S[1, 2] = 3.0
+    A = np.ascontiguousarray(S.T)[None]
+    info = oracle.potf2(A, 5, "L")
+    assert info[0] == 2
+    M = A[0].T
+    assert M[1, 1] == -1.0 and M[2, 1] == 0.0  # t stored, column scaled by 0
+    assert M[2, 2] == 3.0 and M[4, 4] == 4.0 and M[3, 3] == 0.0
+
+
+@needs_lapack
+@pytest.mark.parametrize("n,lda,ldb,nrhs", [(10, 10, 10, 2), (20, 20, 20, 20), (30, 50, 30, 5),
+                                            (50, 60, 60, 1)])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_potrs_and_posv_vs_lapack(n, lda, ldb, nrhs, uplo):
+    """potrs equals LAPACK xPOTRS within n*eps (testing_potrs.hpp:182-200 uses the
+    vector-induced infinity norm); posv = potf2 + potrs with B unchanged when
+    info != 0 (roclapack_posv.hpp:150-168)."""
+    A = wl.reference_potrf(n, lda, 4, singular=True)
+    B = _reference_rhs = np.random.default_rng(n).uniform(1, 10, (4, nrhs, ldb))
+    F = A.copy()
+    info = oracle.potf2(F, n, uplo)
+    G = A.copy()
+    lapack.potrf(G, n, uplo)
+    for b in range(4):
+        if info[b] != 0:
+            continue
+        X, Y = B[b:b + 1].copy(), B[b:b + 1].copy()
+        oracle.potrs(F[b:b + 1], n, X, uplo)
+        lapack.potrs(G[b:b + 1], n, Y, uplo)
+        err = np.abs(X[0, :, :n] - Y[0, :, :n]).max() / np.abs(Y[0, :, :n]).max()
+        assert err <= n * EPS[np.float64]
+        M = A[b, :n, :n].T
+        S = np.tril(M) + np.tril(M, -1).T if uplo == "L" else np.triu(M) + np.triu(M, 1).T
+        assert np.abs(S @ X[0, :, :n].T - B[b, :, :n].T).max() <= 1e3 * n * EPS[np.float64] * np.abs(S).max()
+    Ap, Bp = A.copy(), B.copy()
+    info_p = oracle.posv(Ap, n, Bp, uplo)
+    np.testing.assert_array_equal(info_p, info)
+    for b in range(4):
+        if info[b] != 0:
+            np.testing.assert_array_equal(Bp[b], B[b])
