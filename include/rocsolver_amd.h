@@ -290,6 +290,92 @@ rocblas_status rocsolver_dgetri_npvt_outofplace_strided_batched(rocblas_handle h
     const rocblas_int ldc, const rocblas_stride strideC, rocblas_int* info,
     const rocblas_int batch_count);
 
+/* --------------------------------------------------------------- CHOLESKY
+ * potf2 / potrf: Cholesky factorisation of the uplo triangle in place (lower:
+ * A = L L^T, upper: A = U^T U); info = first non-positive pivot (1-based), at
+ * which the factorisation stores the pivot, scales the rest of its column
+ * (row) by 0 and goes on, as the reference does.  potrs: solve A X = B with
+ * the factor.  posv: potrf + potrs, B unchanged where info != 0.  No
+ * workspace.  Reference declarations (rocsolver-functions.h, s / d): potf2
+ * :9959/:9966, potf2_batched :10031/:10039, potf2_strided_batched
+ * :10111/:10120, potrf :10188/:10195, potrf_batched :10260/:10268,
+ * potrf_strided_batched :10340/:10349, potrs :10426/:10435, potrs_batched
+ * :10517/:10527, potrs_strided_batched :10620/:10632, posv :10718/:10728,
+ * posv_batched :10812/:10823, posv_strided_batched :10918/:10931;
+ * implementation library/src/lapack/roclapack_{potf2,potrf,potrs,posv}*. */
+rocblas_status rocsolver_spotf2(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* A, const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_spotf2_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* const A[], const rocblas_int lda, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_spotf2_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_spotrf(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* A, const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_spotrf_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* const A[], const rocblas_int lda, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_spotrf_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, float* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_spotrs(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, float* A, const rocblas_int lda, float* B,
+    const rocblas_int ldb);
+rocblas_status rocsolver_spotrs_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, float* const A[], const rocblas_int lda,
+    float* const B[], const rocblas_int ldb, const rocblas_int batch_count);
+rocblas_status rocsolver_spotrs_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, float* A, const rocblas_int lda,
+    const rocblas_stride strideA, float* B, const rocblas_int ldb, const rocblas_stride strideB,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_sposv(rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,
+    const rocblas_int nrhs, float* A, const rocblas_int lda, float* B, const rocblas_int ldb,
+    rocblas_int* info);
+rocblas_status rocsolver_sposv_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, float* const A[], const rocblas_int lda,
+    float* const B[], const rocblas_int ldb, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_sposv_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, float* A, const rocblas_int lda,
+    const rocblas_stride strideA, float* B, const rocblas_int ldb, const rocblas_stride strideB,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dpotf2(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* A, const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_dpotf2_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* const A[], const rocblas_int lda, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_dpotf2_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dpotrf(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* A, const rocblas_int lda, rocblas_int* info);
+rocblas_status rocsolver_dpotrf_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* const A[], const rocblas_int lda, rocblas_int* info,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_dpotrf_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, double* A, const rocblas_int lda, const rocblas_stride strideA,
+    rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dpotrs(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, double* A, const rocblas_int lda, double* B,
+    const rocblas_int ldb);
+rocblas_status rocsolver_dpotrs_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, double* const A[], const rocblas_int lda,
+    double* const B[], const rocblas_int ldb, const rocblas_int batch_count);
+rocblas_status rocsolver_dpotrs_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, double* A, const rocblas_int lda,
+    const rocblas_stride strideA, double* B, const rocblas_int ldb, const rocblas_stride strideB,
+    const rocblas_int batch_count);
+rocblas_status rocsolver_dposv(rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,
+    const rocblas_int nrhs, double* A, const rocblas_int lda, double* B, const rocblas_int ldb,
+    rocblas_int* info);
+rocblas_status rocsolver_dposv_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, double* const A[], const rocblas_int lda,
+    double* const B[], const rocblas_int ldb, rocblas_int* info, const rocblas_int batch_count);
+rocblas_status rocsolver_dposv_strided_batched(rocblas_handle handle, const rocblas_fill uplo,
+    const rocblas_int n, const rocblas_int nrhs, double* A, const rocblas_int lda,
+    const rocblas_stride strideA, double* B, const rocblas_int ldb, const rocblas_stride strideB,
+    rocblas_int* info, const rocblas_int batch_count);
+
 /* ------------------------------------------------------------------ LASWP
  * Row interchanges driven by ipiv[k1-1 .. k2-1] (incx = +-1).
  * Reference: rocsolver-functions.h:198/207;

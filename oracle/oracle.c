@@ -30,7 +30,9 @@
  *  potf2   library/src/lapack/roclapack_potf2.hpp:10-37,160-250 (left-looking:
  *          t = a_jj - dot; t <= 0 -> info = first j+1, a_jj = t, scale 0, and
  *          the factorisation goes on; else a_jj = sqrt(t), then the rest of
- *          column j (lower) / row j (upper) minus a gemv, times 1/a_jj)
+ *          column j (lower) / row j (upper) minus a gemv, times 1/a_jj).  The
+ *          dot and gemv sums (rocBLAS reductions of unspecified order there)
+ *          are taken here term by term in increasing k, one fma each
  *  potrs   library/src/lapack/roclapack_potrs.hpp:121-148 (L then L^T, or U^T
  *          then U, non-unit triangular solves)
  *  posv    library/src/lapack/roclapack_posv.hpp:140-170 (potrf, potrs, B
@@ -301,10 +303,10 @@ DEF_GETRI(float, s, fmaf)
         int info = 0;                                                                   \
         for(int j = 0; j < n; ++j)                                                      \
         {                                                                               \
-            T dot = (T)0;                                                               \
+            /* t = a_jj - sum_k l_jk^2, subtracted term by term in k order */            \
+            T t = LEL(j, j);                                                            \
             for(int k = 0; k < j; ++k)                                                  \
-                dot = FMA(LEL(j, k), LEL(j, k), dot);                                   \
-            T t = LEL(j, j) - dot;                                                      \
+                t = FMA(-LEL(j, k), LEL(j, k), t);                                      \
             T r;                                                                        \
             if(t <= (T)0)                                                               \
             {                                                                           \

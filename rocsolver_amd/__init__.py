@@ -102,6 +102,18 @@ SIGNATURES = {
     "gesv_batched": [_P, _I, _I, _P, _I, _P, _S, _P, _I, _P, _I],
     "gesv_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _S, _P, _I, _S, _P, _I],
     "laswp": [_P, _I, _P, _I, _I, _I, _P, _I],
+    "potf2": [_P, _I, _I, _P, _I, _P],
+    "potf2_batched": [_P, _I, _I, _P, _I, _P, _I],
+    "potf2_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _I],
+    "potrf": [_P, _I, _I, _P, _I, _P],
+    "potrf_batched": [_P, _I, _I, _P, _I, _P, _I],
+    "potrf_strided_batched": [_P, _I, _I, _P, _I, _S, _P, _I],
+    "potrs": [_P, _I, _I, _I, _P, _I, _P, _I],
+    "potrs_batched": [_P, _I, _I, _I, _P, _I, _P, _I, _I],
+    "potrs_strided_batched": [_P, _I, _I, _I, _P, _I, _S, _P, _I, _S, _I],
+    "posv": [_P, _I, _I, _I, _P, _I, _P, _I, _P],
+    "posv_batched": [_P, _I, _I, _I, _P, _I, _P, _I, _P, _I],
+    "posv_strided_batched": [_P, _I, _I, _I, _P, _I, _S, _P, _I, _S, _P, _I],
     "getri": [_P, _I, _P, _I, _P, _P],
     "getri_batched": [_P, _I, _P, _I, _P, _S, _P, _I],
     "getri_strided_batched": [_P, _I, _P, _I, _S, _P, _S, _P, _I],
@@ -306,6 +318,55 @@ def getri(handle: Handle, LU, ipiv=None, C=None, form: str = "strided"):
         if C is not None:
             args += [C.data_ptr(), C.shape[2], C.stride(0)]
         check(name + "_strided_batched", handle.value, n, *args, info.data_ptr(), bc)
+    return info
+
+
+FILL_UPPER, FILL_LOWER = 121, 122  # rocblas_fill
+
+
+def potrf(handle: Handle, A, uplo: str = "L", form: str = "strided", unblocked: bool = False):
+    """Cholesky factorisation of the uplo triangle of every A (bc, n, lda) in
+    place (potrf, or potf2 with unblocked=True).  form: "strided", "batched" or
+    "plain" (bc must be 1).  Returns info (bc,) int32 on the device."""
+    import torch
+    bc, n, lda = A.shape
+    p = _prefix(A)
+    u = FILL_UPPER if uplo.upper() == "U" else FILL_LOWER
+    info = torch.zeros(bc, dtype=torch.int32, device=A.device)
+    name = f"rocsolver_{p}" + ("potf2" if unblocked else "potrf")
+    if form == "plain":
+        assert bc == 1
+        check(name, handle.value, u, n, A.data_ptr(), lda, info.data_ptr())
+    elif form == "batched":
+        pa = pointer_array(A)
+        check(name + "_batched", handle.value, u, n, pa.data_ptr(), lda, info.data_ptr(), bc)
+    else:
+        check(name + "_strided_batched", handle.value, u, n, A.data_ptr(), lda, n * lda,
+              info.data_ptr(), bc)
+    return info
+
+
+def potrs(handle: Handle, F, B, uplo: str = "L", form: str = "strided", posv: bool = False):
+    """Solve A X = B in place for B (bc, nrhs, ldb) with the Cholesky factor F
+    (potrs), or factor F first (posv: returns info)."""
+    import torch
+    bc, n, lda = F.shape
+    _, nrhs, ldb = B.shape
+    p = _prefix(F)
+    u = FILL_UPPER if uplo.upper() == "U" else FILL_LOWER
+    name = f"rocsolver_{p}" + ("posv" if posv else "potrs")
+    info = torch.zeros(bc, dtype=torch.int32, device=F.device) if posv else None
+    tail = [info.data_ptr()] if posv else []
+    if form == "plain":
+        assert bc == 1
+        check(name, handle.value, u, n, nrhs, F.data_ptr(), lda, B.data_ptr(), ldb, *tail)
+    elif form == "batched":
+        pa, pb = pointer_array(F), pointer_array(B)
+        check(name + "_batched", handle.value, u, n, nrhs, pa.data_ptr(), lda, pb.data_ptr(), ldb,
+              *tail, bc)
+    else:
+        check(name + "_strided_batched", handle.value, u, n, nrhs, F.data_ptr(), lda, n * lda,
+              B.data_ptr(), ldb, nrhs * ldb, *tail, bc)
     return info
 
 

@@ -195,6 +195,75 @@ rocblas_status getri_impl(rocblas_handle handle, int n, Batch<T> A, bool hasA, i
     return getri_solve<T>(stream, n, W, n, ipiv, strideP, A, lda, info, bc, pivot);
 }
 
+// Cholesky family: argument checks of roclapack_potf2.hpp:95-121 (potf2 /
+// potrf), roclapack_potrs.hpp:20-46 and roclapack_posv.hpp:20-48: uplo value,
+// then sizes, then (unless querying) pointers; no workspace is needed.
+inline rocblas_status check_uplo(rocblas_fill uplo, bool* up)
+{
+    if(uplo != rocblas_fill_upper && uplo != rocblas_fill_lower)
+        return rocblas_status_invalid_value;
+    *up = uplo == rocblas_fill_upper;
+    return rocblas_status_continue;
+}
+
+template <typename T>
+rocblas_status potrf_impl(rocblas_handle handle, rocblas_fill uplo, int n, Batch<T> A, bool hasA,
+                          int lda, int* info, int bc)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    bool up;
+    rocblas_status st = check_uplo(uplo, &up);
+    if(st != rocblas_status_continue)
+        return st;
+    if(n < 0 || lda < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && ((n && !hasA) || (bc && !info)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    return potrf<T>(stream, up, n, A, lda, info, bc);
+}
+
+template <typename T>
+rocblas_status potrs_impl(rocblas_handle handle, rocblas_fill uplo, int n, int nrhs, Batch<T> A,
+                          bool hasA, int lda, Batch<T> B, bool hasB, int ldb, int* info, int bc,
+                          bool posv)
+{
+    if(!handle)
+        return rocblas_status_invalid_handle;
+    bool up;
+    rocblas_status st = check_uplo(uplo, &up);
+    if(st != rocblas_status_continue)
+        return st;
+    if(n < 0 || nrhs < 0 || lda < n || ldb < n || bc < 0)
+        return rocblas_status_invalid_size;
+    const bool query = rocblas_is_device_memory_size_query(handle);
+    if(!query && ((n && !hasA) || (int64_t(nrhs) * n && !hasB) || (posv && bc && !info)))
+        return rocblas_status_invalid_pointer;
+    if(query)
+        return report_no_workspace(handle);
+    if(bc == 0)
+        return rocblas_status_success;
+    hipStream_t stream;
+    rocblas_get_stream(handle, &stream);
+    if(!posv)
+        return potrs<T>(stream, up, n, nrhs, A, lda, B, ldb, bc, nullptr);
+    // posv: info reset, then nothing more when A or B is empty
+    // (roclapack_posv.hpp:135-142); B is left unchanged where info != 0
+    if(n == 0 || nrhs == 0)
+        return reset_info(stream, info, bc);
+    st = potrf<T>(stream, up, n, A, lda, info, bc);
+    if(st != rocblas_status_success)
+        return st;
+    return potrs<T>(stream, up, n, nrhs, A, lda, B, ldb, bc, info);
+}
+
 constexpr int DSGESV_ITERMAX = 30; // LAPACK dsgesv ITERMAX
 
 rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, const double* A,
@@ -660,6 +729,101 @@ RSA_ADJOINT(d, double)
     }
 RSA_GETRI(s, float)
 RSA_GETRI(d, double)
+
+#define RSA_CHOLESKY(P, T)                                                                      \
+    rocblas_status rocsolver_##P##potf2(rocblas_handle handle, const rocblas_fill uplo,         \
+                                        const rocblas_int n, T* A, const rocblas_int lda,       \
+                                        rocblas_int* info)                                      \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, strided(A, 0), A != nullptr, lda, info, 1);       \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potf2_batched(rocblas_handle handle, const rocblas_fill uplo, \
+                                                const rocblas_int n, T* const A[],              \
+                                                const rocblas_int lda, rocblas_int* info,       \
+                                                const rocblas_int batch_count)                  \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, arrayed(A), A != nullptr, lda, info, batch_count); \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potf2_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n, T* A,              \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
+        const rocblas_int batch_count)                                                          \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, strided(A, strideA), A != nullptr, lda, info,     \
+                             batch_count);                                                      \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrf(rocblas_handle handle, const rocblas_fill uplo,         \
+                                        const rocblas_int n, T* A, const rocblas_int lda,       \
+                                        rocblas_int* info)                                      \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, strided(A, 0), A != nullptr, lda, info, 1);       \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrf_batched(rocblas_handle handle, const rocblas_fill uplo, \
+                                                const rocblas_int n, T* const A[],              \
+                                                const rocblas_int lda, rocblas_int* info,       \
+                                                const rocblas_int batch_count)                  \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, arrayed(A), A != nullptr, lda, info, batch_count); \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrf_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n, T* A,              \
+        const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
+        const rocblas_int batch_count)                                                          \
+    {                                                                                           \
+        return potrf_impl<T>(handle, uplo, n, strided(A, strideA), A != nullptr, lda, info,     \
+                             batch_count);                                                      \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrs(rocblas_handle handle, const rocblas_fill uplo,         \
+                                        const rocblas_int n, const rocblas_int nrhs, T* A,      \
+                                        const rocblas_int lda, T* B, const rocblas_int ldb)     \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, 0), A != nullptr, lda,           \
+                             strided(B, 0), B != nullptr, ldb, nullptr, 1, false);              \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrs_batched(                                                \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,                    \
+        const rocblas_int nrhs, T* const A[], const rocblas_int lda, T* const B[],              \
+        const rocblas_int ldb, const rocblas_int batch_count)                                   \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, arrayed(A), A != nullptr, lda, arrayed(B),  \
+                             B != nullptr, ldb, nullptr, batch_count, false);                   \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##potrs_strided_batched(                                        \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,                    \
+        const rocblas_int nrhs, T* A, const rocblas_int lda, const rocblas_stride strideA, T* B, \
+        const rocblas_int ldb, const rocblas_stride strideB, const rocblas_int batch_count)     \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, strideA), A != nullptr, lda,     \
+                             strided(B, strideB), B != nullptr, ldb, nullptr, batch_count,      \
+                             false);                                                            \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##posv(rocblas_handle handle, const rocblas_fill uplo,          \
+                                       const rocblas_int n, const rocblas_int nrhs, T* A,       \
+                                       const rocblas_int lda, T* B, const rocblas_int ldb,      \
+                                       rocblas_int* info)                                       \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, 0), A != nullptr, lda,           \
+                             strided(B, 0), B != nullptr, ldb, info, 1, true);                  \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##posv_batched(                                                 \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,                    \
+        const rocblas_int nrhs, T* const A[], const rocblas_int lda, T* const B[],              \
+        const rocblas_int ldb, rocblas_int* info, const rocblas_int batch_count)                \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, arrayed(A), A != nullptr, lda, arrayed(B),  \
+                             B != nullptr, ldb, info, batch_count, true);                       \
+    }                                                                                           \
+    rocblas_status rocsolver_##P##posv_strided_batched(                                         \
+        rocblas_handle handle, const rocblas_fill uplo, const rocblas_int n,                    \
+        const rocblas_int nrhs, T* A, const rocblas_int lda, const rocblas_stride strideA, T* B, \
+        const rocblas_int ldb, const rocblas_stride strideB, rocblas_int* info,                 \
+        const rocblas_int batch_count)                                                          \
+    {                                                                                           \
+        return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, strideA), A != nullptr, lda,     \
+                             strided(B, strideB), B != nullptr, ldb, info, batch_count, true);  \
+    }
+RSA_CHOLESKY(s, float)
+RSA_CHOLESKY(d, double)
 
 #define RSA_VBATCHED(P, T)                                                                      \
     rocblas_status rocsolver_amd_##P##getrf_vbatched(                                           \

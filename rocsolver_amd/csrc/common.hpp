@@ -171,6 +171,16 @@ __device__ __forceinline__ float fma_t(float a, float b, float c)
     return __builtin_fmaf(a, b, c);
 }
 
+// correctly rounded square roots (the IEEE operation, as the host's)
+__device__ __forceinline__ double sqrt_t(double a)
+{
+    return __builtin_sqrt(a);
+}
+__device__ __forceinline__ float sqrt_t(float a)
+{
+    return __builtin_sqrtf(a);
+}
+
 __device__ __forceinline__ double abs_t(double a)
 {
     return __builtin_fabs(a);

@@ -137,6 +137,17 @@ template <typename T>
 rocblas_status copy_square(hipStream_t stream, int n, Batch<T> S, int lds, Batch<T> D, int ldd,
                            int bc);
 
+// ---------------------------------------------------------------- Cholesky
+// In-place factorisation of the lower (up = false: A = L L^T) or upper
+// (A = U^T U) triangle; info = first non-positive pivot (1-based) or 0.
+template <typename T>
+rocblas_status potrf(hipStream_t stream, bool up, int n, Batch<T> A, int lda, int* info, int bc);
+// Solve A X = B with that factor, B overwritten by X; systems with
+// skip[b] != 0 are left untouched.
+template <typename T>
+rocblas_status potrs(hipStream_t stream, bool up, int n, int nrhs, Batch<T> A, int lda,
+                     Batch<T> B, int ldb, int bc, const int* skip);
+
 // --------------------------------------------------------------- utilities
 rocblas_status reset_info(hipStream_t stream, int* info, int bc);
 

@@ -140,9 +140,10 @@ __device__ __forceinline__ void apply_inv_block(TL (*sx)[NR], const TL (*sL)[GBS
 }
 
 // Forward substitution with the lower triangle of op(A): unit diagonal for
-// op N (L), non-unit for op T (U^T).  sx: LDS [2][GBS][NR]; sL: LDS
-// [GBS][GBS+1].  INV (op N only): the diagonal blocks hold their inverses.
-template <typename TL, int NR, bool TRANS, bool INV = false>
+// op N (L), non-unit for op T (U^T); DIAG = 1 forces a non-unit diagonal
+// (Cholesky factors).  sx: LDS [2][GBS][NR]; sL: LDS [GBS][GBS+1].  INV
+// (op N only): the diagonal blocks hold their inverses.
+template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1>
 __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
@@ -172,7 +173,7 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
         if constexpr(INV)
             apply_inv_block<TL, NR, true>(sx[par], sL);
         else
-            solve_diag_block<TL, NR, true, TRANS>(r0, n, sx[par], sL);
+            solve_diag_block<TL, NR, true, (DIAG < 0 ? TRANS : DIAG == 1)>(r0, n, sx[par], sL);
         __syncthreads();
         if(owner)
         {
@@ -196,8 +197,9 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
 }
 
 // Backward substitution with the upper triangle of op(A): non-unit for op N
-// (U), unit diagonal for op T (L^T).  INV as for solve_lower_rows.
-template <typename TL, int NR, bool TRANS, bool INV = false>
+// (U), unit diagonal for op T (L^T); DIAG = 1 forces a non-unit diagonal.
+// INV as for solve_lower_rows.
+template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1>
 __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
@@ -230,7 +232,7 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
         if constexpr(INV)
             apply_inv_block<TL, NR, false>(sx[par], sL);
         else
-            solve_diag_block<TL, NR, false, !TRANS>(r0, n, sx[par], sL);
+            solve_diag_block<TL, NR, false, (DIAG < 0 ? !TRANS : DIAG == 1)>(r0, n, sx[par], sL);
         __syncthreads();
         if(owner)
         {

@@ -160,3 +160,22 @@ def test_getri_argument_checks(p):
     assert ra.call(g, DUMMY, 4, DUMMY, 4, 16, DUMMY, 4, DUMMY, 3, 16, DUMMY, 1) == 4  # ldc < n
     assert ra.call(f"rocsolver_{p}getri_npvt_outofplace_batched", DUMMY, 4, DUMMY, 4, DUMMY, 2,
                    DUMMY, 1) == 4
+
+
+@pytest.mark.parametrize("p", ["s", "d"])
+def test_cholesky_argument_checks(p):
+    """potf2/potrf (roclapack_potf2.hpp:95-121), potrs (roclapack_potrs.hpp:20-46),
+    posv (roclapack_posv.hpp:20-48): null handle, then uplo value, then sizes."""
+    L = 122  # rocblas_fill_lower
+    for name in ("potf2_strided_batched", "potrf_strided_batched"):
+        f = f"rocsolver_{p}{name}"
+        assert ra.call(f, None, L, 4, DUMMY, 4, 16, DUMMY, 1) == 1
+        assert ra.call(f, DUMMY, 123, -1, DUMMY, 4, 16, DUMMY, 1) == 11  # value before size
+        for n, lda, bc in [(-1, 4, 1), (4, 3, 1), (4, 4, -1)]:
+            assert ra.call(f, DUMMY, L, n, DUMMY, lda, 16, DUMMY, bc) == 4
+    for name, extra in (("potrs_strided_batched", []), ("posv_strided_batched", [DUMMY])):
+        f = f"rocsolver_{p}{name}"
+        assert ra.call(f, DUMMY, 0, 4, 1, DUMMY, 4, 16, DUMMY, 4, 4, *extra, 1) == 11
+        for n, nrhs, lda, ldb, bc in [(-1, 1, 4, 4, 1), (4, -1, 4, 4, 1), (10, 1, 2, 10, 1),
+                                      (10, 1, 10, 2, 1), (4, 1, 4, 4, -1)]:
+            assert ra.call(f, DUMMY, L, n, nrhs, DUMMY, lda, 16, DUMMY, ldb, 4, *extra, bc) == 4
