@@ -541,6 +541,31 @@ rocblas_status rocsolver_amd_vbatched_set_tuning(const rocblas_int* params,
                                                  const rocblas_int len);
 rocblas_status rocsolver_amd_vbatched_get_tuning(rocblas_int* params, const rocblas_int len);
 
+/* ---------------------------------------------------------------- logging
+ * Multi-level logging of the rocsolver_* entry points: trace trees, bench
+ * lines ("./rocsolver-bench -f <function> -r <precision> <arguments>") and
+ * profile totals of the internal stages, per the layer-mode bits 0x1 trace,
+ * 0x2 bench, 0x4 profile (profile synchronises the handle's stream after
+ * every stage).  log_begin reads ROCSOLVER_LAYER and ROCSOLVER_LEVELS and
+ * opens ROCSOLVER_LOG_{TRACE,BENCH,PROFILE}_PATH or ROCSOLVER_LOG_PATH
+ * (default stderr); every call without an active session, log_begin with
+ * one, and log_end with calls still in flight return
+ * rocblas_status_internal_error.  Reference: rocsolver-functions.h:66
+ * (log_begin), :75 (log_end), :86 (log_set_layer_mode), :98
+ * (log_set_max_levels), :108 (log_restore_defaults), :113
+ * (log_write_profile), :119 (log_flush_profile); rocsolver-extra-types.h:13
+ * (rocblas_layer_mode_flags); library/src/common/rocsolver_logger.cpp. */
+#ifndef ROCSOLVER_EXTRA_TYPES_H
+typedef uint32_t rocblas_layer_mode_flags;
+#endif
+rocblas_status rocsolver_log_begin(void);
+rocblas_status rocsolver_log_end(void);
+rocblas_status rocsolver_log_set_layer_mode(const rocblas_layer_mode_flags layer_mode);
+rocblas_status rocsolver_log_set_max_levels(const rocblas_int max_levels);
+rocblas_status rocsolver_log_restore_defaults(void);
+rocblas_status rocsolver_log_write_profile(void);
+rocblas_status rocsolver_log_flush_profile(void);
+
 /* ---------------------------------------------------------------- version
  * Reference: rocsolver-functions.h:34 and :45; library/src/common/buildinfo.cpp. */
 rocblas_status rocsolver_get_version_string(char* buf, size_t len);

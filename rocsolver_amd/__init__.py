@@ -147,6 +147,13 @@ EXTENSIONS = {
     "rocsolver_amd_dxgesv_strided_batched": [_P, _I, _I, _I, _P, _I, _S, _P, _I, _S, _P, _I, _S,
                                              _P, _P, _P, _I],
     "rocsolver_amd_vbatched_set_tuning": [_P, _I],
+    "rocsolver_log_begin": [],
+    "rocsolver_log_end": [],
+    "rocsolver_log_set_layer_mode": [ctypes.c_uint32],
+    "rocsolver_log_set_max_levels": [_I],
+    "rocsolver_log_restore_defaults": [],
+    "rocsolver_log_write_profile": [],
+    "rocsolver_log_flush_profile": [],
     "rocsolver_amd_vbatched_get_tuning": [_P, _I],
 }
 

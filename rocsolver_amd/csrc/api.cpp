@@ -12,6 +12,7 @@
 // rocauxiliary_laswp.hpp:66-94 and rocauxiliary_laswp.cpp:7-42 (laswp).
 #include "../../include/rocsolver_amd.h"
 #include "internal.hpp"
+#include "logging.hpp"
 
 #include <rocblas/internal/rocblas_device_malloc.hpp>
 
@@ -50,9 +51,16 @@ rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool 
     if(m == 0 || n == 0)
         return reset_info(stream, info, bc);
     if(m <= SMALL_MAX && n <= SMALL_MAX)
+    {
+        RSA_LOG_STAGE(handle, "getrf_small", "m", m, "n", n, "pivot", int(pivot), "bc", bc);
         return getrf_small<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
+    }
     if(pivot && m == n && n <= FUSED_MAX && fits_buffer<T>(lda, n))
+    {
+        RSA_LOG_STAGE(handle, "getrf_fused", "n", n, "bc", bc);
         return lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
+    }
+    RSA_LOG_STAGE(handle, "getrf_blocked", "m", m, "n", n, "pivot", int(pivot), "bc", bc);
     return getrf_blocked<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
 }
 
@@ -89,10 +97,17 @@ rocblas_status getrs_impl(rocblas_handle handle, rocblas_operation trans, int n,
     hipStream_t stream;
     rocblas_get_stream(handle, &stream);
     if(n <= SMALL_MAX)
+    {
+        RSA_LOG_STAGE(handle, "getrs_small", "n", n, "nrhs", nrhs, "trans", trans, "bc", bc);
         return getrs_small<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true);
+    }
     if(n <= FUSED_MAX && fits_buffer<T>(lda, n) && fits_buffer<T>(ldb, nrhs))
+    {
+        RSA_LOG_STAGE(handle, "getrs_fused", "n", n, "nrhs", nrhs, "trans", trans, "bc", bc);
         return getrs_fused<T, T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb, bc,
                                  nullptr, false);
+    }
+    RSA_LOG_STAGE(handle, "getrs_blocked", "n", n, "nrhs", nrhs, "trans", trans, "bc", bc);
     return getrs_blocked<T>(stream, tr, n, nrhs, A, lda, ipiv, strideP, B, ldb, bc, true,
                             nullptr);
 }
@@ -449,6 +464,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                         const rocblas_int n, T* A, const rocblas_int lda,       \
                                         rocblas_int* ipiv, rocblas_int* info)                   \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf", "-m", m, "-n", n, "--lda", lda);                    \
         return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, ipiv, 0, info, 1, \
                              true);                                                             \
     }                                                                                           \
@@ -458,6 +474,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                                 const rocblas_stride strideP,                   \
                                                 rocblas_int* info, const rocblas_int bc)        \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf_batched", "-m", m, "-n", n, "--lda", lda, "--strideP", strideP, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,     \
                              info, bc, true);                                                   \
     }                                                                                           \
@@ -466,6 +483,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,                 \
         const rocblas_stride strideP, rocblas_int* info, const rocblas_int bc)                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf_strided_batched", "-m", m, "-n", n, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, ipiv,        \
                              strideP, info, bc, true);                                          \
     }                                                                                           \
@@ -473,6 +491,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                         const rocblas_int n, T* A, const rocblas_int lda,       \
                                         rocblas_int* ipiv, rocblas_int* info)                   \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getf2", "-m", m, "-n", n, "--lda", lda);                    \
         return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, ipiv, 0, info, 1, \
                              true);                                                             \
     }                                                                                           \
@@ -482,6 +501,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                                 const rocblas_stride strideP,                   \
                                                 rocblas_int* info, const rocblas_int bc)        \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getf2_batched", "-m", m, "-n", n, "--lda", lda, "--strideP", strideP, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,     \
                              info, bc, true);                                                   \
     }                                                                                           \
@@ -490,6 +510,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int lda, const rocblas_stride strideA, rocblas_int* ipiv,                 \
         const rocblas_stride strideP, rocblas_int* info, const rocblas_int bc)                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getf2_strided_batched", "-m", m, "-n", n, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, ipiv,        \
                              strideP, info, bc, true);                                          \
     }                                                                                           \
@@ -497,6 +518,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                              const rocblas_int n, T* A, const rocblas_int lda,  \
                                              rocblas_int* info)                                 \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf_npvt", "-m", m, "-n", n, "--lda", lda);               \
         return getrf_impl<T>(handle, m, n, strided(A, 0), A != nullptr, lda, nullptr, 0, info,  \
                              1, false);                                                         \
     }                                                                                           \
@@ -505,6 +527,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                                      const rocblas_int lda, rocblas_int* info,  \
                                                      const rocblas_int bc)                      \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf_npvt_batched", "-m", m, "-n", n, "--lda", lda, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, arrayed<T>(A), A != nullptr, lda, nullptr, 0, info,  \
                              bc, false);                                                        \
     }                                                                                           \
@@ -513,6 +536,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
         const rocblas_int bc)                                                                   \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrf_npvt_strided_batched", "-m", m, "-n", n, "--lda", lda, "--strideA", strideA, "--batch_count", bc);\
         return getrf_impl<T>(handle, m, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,  \
                              info, bc, false);                                                  \
     }
@@ -523,6 +547,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                         const rocblas_int lda, const rocblas_int* ipiv, T* B,   \
                                         const rocblas_int ldb)                                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrs", "--trans", trans, "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb);\
         return getrs_impl<T>(handle, trans, n, nrhs, strided(A, 0), A != nullptr, lda, ipiv, 0, \
                              strided(B, 0), B != nullptr, ldb, 1);                              \
     }                                                                                           \
@@ -531,6 +556,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int nrhs, T* const A[], const rocblas_int lda, const rocblas_int* ipiv,    \
         const rocblas_stride strideP, T* const B[], const rocblas_int ldb, const rocblas_int bc) \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrs_batched", "--trans", trans, "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideP", strideP, "--ldb", ldb, "--batch_count", bc);\
         return getrs_impl<T>(handle, trans, n, nrhs, arrayed<T>(A), A != nullptr, lda, ipiv,    \
                              strideP, arrayed<T>(B), B != nullptr, ldb, bc);                    \
     }                                                                                           \
@@ -540,6 +566,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int* ipiv, const rocblas_stride strideP, T* B, const rocblas_int ldb,     \
         const rocblas_stride strideB, const rocblas_int bc)                                     \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getrs_strided_batched", "--trans", trans, "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--ldb", ldb, "--strideB", strideB, "--batch_count", bc);\
         return getrs_impl<T>(handle, trans, n, nrhs, strided(A, strideA), A != nullptr, lda,    \
                              ipiv, strideP, strided(B, strideB), B != nullptr, ldb, bc);        \
     }                                                                                           \
@@ -548,6 +575,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                        rocblas_int* ipiv, T* B, const rocblas_int ldb,          \
                                        rocblas_int* info)                                       \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "gesv", "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb);\
         return gesv_impl<T>(handle, n, nrhs, strided(A, 0), A != nullptr, lda, ipiv, 0,         \
                             strided(B, 0), B != nullptr, ldb, info, 1);                         \
     }                                                                                           \
@@ -556,6 +584,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_int lda, rocblas_int* ipiv, const rocblas_stride strideP, T* const B[],   \
         const rocblas_int ldb, rocblas_int* info, const rocblas_int bc)                         \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "gesv_batched", "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideP", strideP, "--ldb", ldb, "--batch_count", bc);\
         return gesv_impl<T>(handle, n, nrhs, arrayed<T>(A), A != nullptr, lda, ipiv, strideP,   \
                             arrayed<T>(B), B != nullptr, ldb, info, bc);                        \
     }                                                                                           \
@@ -565,6 +594,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
         const rocblas_stride strideP, T* B, const rocblas_int ldb,                              \
         const rocblas_stride strideB, rocblas_int* info, const rocblas_int bc)                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "gesv_strided_batched", "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--ldb", ldb, "--strideB", strideB, "--batch_count", bc);\
         return gesv_impl<T>(handle, n, nrhs, strided(A, strideA), A != nullptr, lda, ipiv,      \
                             strideP, strided(B, strideB), B != nullptr, ldb, info, bc);         \
     }                                                                                           \
@@ -573,6 +603,7 @@ rocblas_status dgesv_vbatched_noalloc_impl(rocblas_handle handle, const int* n, 
                                         const rocblas_int k2, const rocblas_int* ipiv,          \
                                         const rocblas_int incx)                                 \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "laswp", "-n", n, "--lda", lda, "--k1", k1, "--k2", k2, "--incx", incx);\
         return laswp_impl<T>(handle, n, A, lda, k1, k2, ipiv, incx);                            \
     }
 
@@ -636,6 +667,7 @@ RSA_ADJOINT(d, double)
                                         const rocblas_int lda, rocblas_int* ipiv,               \
                                         rocblas_int* info)                                      \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri", "-n", n, "--lda", lda);                             \
         return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, ipiv, 0, false,       \
                              strided<T>(nullptr, 0), false, 0, info, 1, true);                  \
     }                                                                                           \
@@ -644,6 +676,7 @@ RSA_ADJOINT(d, double)
                                                 rocblas_int* ipiv, const rocblas_stride strideP, \
                                                 rocblas_int* info, const rocblas_int batch_count) \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_batched", "-n", n, "--lda", lda, "--strideP", strideP, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, ipiv, strideP, false,    \
                              strided<T>(nullptr, 0), false, 0, info, batch_count, true);        \
     }                                                                                           \
@@ -652,12 +685,14 @@ RSA_ADJOINT(d, double)
         const rocblas_stride strideA, rocblas_int* ipiv, const rocblas_stride strideP,          \
         rocblas_int* info, const rocblas_int batch_count)                                       \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_strided_batched", "-n", n, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, ipiv, strideP,   \
                              false, strided<T>(nullptr, 0), false, 0, info, batch_count, true); \
     }                                                                                           \
     rocblas_status rocsolver_##P##getri_npvt(rocblas_handle handle, const rocblas_int n, T* A,  \
                                              const rocblas_int lda, rocblas_int* info)          \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt", "-n", n, "--lda", lda);                        \
         return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, nullptr, 0, false,    \
                              strided<T>(nullptr, 0), false, 0, info, 1, false);                 \
     }                                                                                           \
@@ -666,6 +701,7 @@ RSA_ADJOINT(d, double)
                                                      rocblas_int* info,                         \
                                                      const rocblas_int batch_count)             \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt_batched", "-n", n, "--lda", lda, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, nullptr, 0, false,       \
                              strided<T>(nullptr, 0), false, 0, info, batch_count, false);       \
     }                                                                                           \
@@ -673,6 +709,7 @@ RSA_ADJOINT(d, double)
         rocblas_handle handle, const rocblas_int n, T* A, const rocblas_int lda,                \
         const rocblas_stride strideA, rocblas_int* info, const rocblas_int batch_count)         \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt_strided_batched", "-n", n, "--lda", lda, "--strideA", strideA, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,      \
                              false, strided<T>(nullptr, 0), false, 0, info, batch_count, false); \
     }                                                                                           \
@@ -681,6 +718,7 @@ RSA_ADJOINT(d, double)
                                                    rocblas_int* ipiv, T* C,                     \
                                                    const rocblas_int ldc, rocblas_int* info)    \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_outofplace", "-n", n, "--lda", lda, "--ldc", ldc);    \
         return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, ipiv, 0, true,       \
                              strided(C, 0), C != nullptr, ldc, info, 1, true);                  \
     }                                                                                           \
@@ -689,6 +727,7 @@ RSA_ADJOINT(d, double)
         rocblas_int* ipiv, const rocblas_stride strideP, T* const C[], const rocblas_int ldc,   \
         rocblas_int* info, const rocblas_int batch_count)                                       \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_outofplace_batched", "-n", n, "--lda", lda, "--strideP", strideP, "--ldc", ldc, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, ipiv, strideP, true,     \
                              arrayed(C), C != nullptr, ldc, info, batch_count, true);           \
     }                                                                                           \
@@ -698,6 +737,7 @@ RSA_ADJOINT(d, double)
         const rocblas_int ldc, const rocblas_stride strideC, rocblas_int* info,                 \
         const rocblas_int batch_count)                                                          \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_outofplace_strided_batched", "-n", n, "--lda", lda, "--strideA", strideA, "--strideP", strideP, "--ldc", ldc, "--strideC", strideC, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, ipiv, strideP,   \
                              true, strided(C, strideC), C != nullptr, ldc, info, batch_count,   \
                              true);                                                             \
@@ -708,6 +748,7 @@ RSA_ADJOINT(d, double)
                                                         const rocblas_int ldc,                  \
                                                         rocblas_int* info)                      \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt_outofplace", "-n", n, "--lda", lda, "--ldc", ldc);\
         return getri_impl<T>(handle, n, strided(A, 0), A != nullptr, lda, nullptr, 0, true,     \
                              strided(C, 0), C != nullptr, ldc, info, 1, false);                 \
     }                                                                                           \
@@ -715,6 +756,7 @@ RSA_ADJOINT(d, double)
         rocblas_handle handle, const rocblas_int n, T* const A[], const rocblas_int lda,        \
         T* const C[], const rocblas_int ldc, rocblas_int* info, const rocblas_int batch_count)  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt_outofplace_batched", "-n", n, "--lda", lda, "--ldc", ldc, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, arrayed(A), A != nullptr, lda, nullptr, 0, true,        \
                              arrayed(C), C != nullptr, ldc, info, batch_count, false);          \
     }                                                                                           \
@@ -723,6 +765,7 @@ RSA_ADJOINT(d, double)
         const rocblas_stride strideA, T* C, const rocblas_int ldc, const rocblas_stride strideC, \
         rocblas_int* info, const rocblas_int batch_count)                                       \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "getri_npvt_outofplace_strided_batched", "-n", n, "--lda", lda, "--strideA", strideA, "--ldc", ldc, "--strideC", strideC, "--batch_count", batch_count);\
         return getri_impl<T>(handle, n, strided(A, strideA), A != nullptr, lda, nullptr, 0,      \
                              true, strided(C, strideC), C != nullptr, ldc, info, batch_count,   \
                              false);                                                            \
@@ -735,6 +778,7 @@ RSA_GETRI(d, double)
                                         const rocblas_int n, T* A, const rocblas_int lda,       \
                                         rocblas_int* info)                                      \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potf2", "--uplo", uplo, "-n", n, "--lda", lda);             \
         return potrf_impl<T>(handle, uplo, n, strided(A, 0), A != nullptr, lda, info, 1);       \
     }                                                                                           \
     rocblas_status rocsolver_##P##potf2_batched(rocblas_handle handle, const rocblas_fill uplo, \
@@ -742,6 +786,7 @@ RSA_GETRI(d, double)
                                                 const rocblas_int lda, rocblas_int* info,       \
                                                 const rocblas_int batch_count)                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potf2_batched", "--uplo", uplo, "-n", n, "--lda", lda, "--batch_count", batch_count);\
         return potrf_impl<T>(handle, uplo, n, arrayed(A), A != nullptr, lda, info, batch_count); \
     }                                                                                           \
     rocblas_status rocsolver_##P##potf2_strided_batched(                                        \
@@ -749,6 +794,7 @@ RSA_GETRI(d, double)
         const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
         const rocblas_int batch_count)                                                          \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potf2_strided_batched", "--uplo", uplo, "-n", n, "--lda", lda, "--strideA", strideA, "--batch_count", batch_count);\
         return potrf_impl<T>(handle, uplo, n, strided(A, strideA), A != nullptr, lda, info,     \
                              batch_count);                                                      \
     }                                                                                           \
@@ -756,6 +802,7 @@ RSA_GETRI(d, double)
                                         const rocblas_int n, T* A, const rocblas_int lda,       \
                                         rocblas_int* info)                                      \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrf", "--uplo", uplo, "-n", n, "--lda", lda);             \
         return potrf_impl<T>(handle, uplo, n, strided(A, 0), A != nullptr, lda, info, 1);       \
     }                                                                                           \
     rocblas_status rocsolver_##P##potrf_batched(rocblas_handle handle, const rocblas_fill uplo, \
@@ -763,6 +810,7 @@ RSA_GETRI(d, double)
                                                 const rocblas_int lda, rocblas_int* info,       \
                                                 const rocblas_int batch_count)                  \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrf_batched", "--uplo", uplo, "-n", n, "--lda", lda, "--batch_count", batch_count);\
         return potrf_impl<T>(handle, uplo, n, arrayed(A), A != nullptr, lda, info, batch_count); \
     }                                                                                           \
     rocblas_status rocsolver_##P##potrf_strided_batched(                                        \
@@ -770,6 +818,7 @@ RSA_GETRI(d, double)
         const rocblas_int lda, const rocblas_stride strideA, rocblas_int* info,                 \
         const rocblas_int batch_count)                                                          \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrf_strided_batched", "--uplo", uplo, "-n", n, "--lda", lda, "--strideA", strideA, "--batch_count", batch_count);\
         return potrf_impl<T>(handle, uplo, n, strided(A, strideA), A != nullptr, lda, info,     \
                              batch_count);                                                      \
     }                                                                                           \
@@ -777,6 +826,7 @@ RSA_GETRI(d, double)
                                         const rocblas_int n, const rocblas_int nrhs, T* A,      \
                                         const rocblas_int lda, T* B, const rocblas_int ldb)     \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrs", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb);\
         return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, 0), A != nullptr, lda,           \
                              strided(B, 0), B != nullptr, ldb, nullptr, 1, false);              \
     }                                                                                           \
@@ -785,6 +835,7 @@ RSA_GETRI(d, double)
         const rocblas_int nrhs, T* const A[], const rocblas_int lda, T* const B[],              \
         const rocblas_int ldb, const rocblas_int batch_count)                                   \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrs_batched", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb, "--batch_count", batch_count);\
         return potrs_impl<T>(handle, uplo, n, nrhs, arrayed(A), A != nullptr, lda, arrayed(B),  \
                              B != nullptr, ldb, nullptr, batch_count, false);                   \
     }                                                                                           \
@@ -793,6 +844,7 @@ RSA_GETRI(d, double)
         const rocblas_int nrhs, T* A, const rocblas_int lda, const rocblas_stride strideA, T* B, \
         const rocblas_int ldb, const rocblas_stride strideB, const rocblas_int batch_count)     \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "potrs_strided_batched", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideA", strideA, "--ldb", ldb, "--strideB", strideB, "--batch_count", batch_count);\
         return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, strideA), A != nullptr, lda,     \
                              strided(B, strideB), B != nullptr, ldb, nullptr, batch_count,      \
                              false);                                                            \
@@ -802,6 +854,7 @@ RSA_GETRI(d, double)
                                        const rocblas_int lda, T* B, const rocblas_int ldb,      \
                                        rocblas_int* info)                                       \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "posv", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb);\
         return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, 0), A != nullptr, lda,           \
                              strided(B, 0), B != nullptr, ldb, info, 1, true);                  \
     }                                                                                           \
@@ -810,6 +863,7 @@ RSA_GETRI(d, double)
         const rocblas_int nrhs, T* const A[], const rocblas_int lda, T* const B[],              \
         const rocblas_int ldb, rocblas_int* info, const rocblas_int batch_count)                \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "posv_batched", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--ldb", ldb, "--batch_count", batch_count);\
         return potrs_impl<T>(handle, uplo, n, nrhs, arrayed(A), A != nullptr, lda, arrayed(B),  \
                              B != nullptr, ldb, info, batch_count, true);                       \
     }                                                                                           \
@@ -819,6 +873,7 @@ RSA_GETRI(d, double)
         const rocblas_int ldb, const rocblas_stride strideB, rocblas_int* info,                 \
         const rocblas_int batch_count)                                                          \
     {                                                                                           \
+        RSA_LOG_TOP(handle, P, "posv_strided_batched", "--uplo", uplo, "-n", n, "--nrhs", nrhs, "--lda", lda, "--strideA", strideA, "--ldb", ldb, "--strideB", strideB, "--batch_count", batch_count);\
         return potrs_impl<T>(handle, uplo, n, nrhs, strided(A, strideA), A != nullptr, lda,     \
                              strided(B, strideB), B != nullptr, ldb, info, batch_count, true);  \
     }
