@@ -280,18 +280,19 @@ def measured_traffic(kernel: str, summary: str = "traffic_mixed_gesv_n256_batch4
 GETRF32_N, GETRF32_BATCH = 32, 131072
 
 
-def bench_getrf32(args) -> int:
+def measure_getrf32(args, with_cpu: bool):
     """Plain FP64 rocsolver_dgetrf_strided_batched at n = 32 (lda 32, stride
     1024, entries U(-1,1)), batch 131072 (2.2 GB of HBM traffic per call, so
     launch latency does not enter the roofline figure).  A step is one call;
     the input is restored before each step with a device copy queued ahead of
     the start event, so the GPU never waits for the host inside the timed
-    region.  Roofline: HBM bytes per matrix = A read + A written + ipiv + info."""
+    region.  Roofline: HBM bytes per matrix = A read + A written + ipiv + info.
+    Returns (seconds per call, GB/s, 1 GiB device-copy GB/s, nonzero info,
+    host-LAPACK baseline or None)."""
     import numpy as np
     import torch
 
     import rocsolver_amd as ra
-    torch.cuda.set_device(0)
     n, bc = GETRF32_N, GETRF32_BATCH
     g = torch.Generator(device="cuda").manual_seed(11)
     A0 = torch.rand((bc, n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
@@ -304,13 +305,13 @@ def bench_getrf32(args) -> int:
         ra.check("rocsolver_dgetrf_strided_batched", handle.value, n, n, A.data_ptr(), n, n * n,
                  ipiv.data_ptr(), n, info.data_ptr(), bc)
     s = torch.cuda.current_stream()
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):
         A.copy_(A0)
         step()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     times = []
-    for _ in range(args.steps):
+    for _ in range(max(args.steps, 3)):
         A.copy_(A0)
         e0.record(s)
         step()
@@ -320,6 +321,8 @@ def bench_getrf32(args) -> int:
     t = float(np.mean(times))
     per = 2 * n * n * 8 + n * 4 + 4
     gbs = bc * per / t / 1e9
+    nonzero = int((info != 0).sum())
+    del A
     # achievable HBM rate on this box, for context: a 1 GiB device-to-device
     # copy (1 GiB read + 1 GiB written), best of 5
     src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
@@ -335,7 +338,7 @@ def bench_getrf32(args) -> int:
     copy_gbs = 2 * src.numel() * 8 / copy_best / 1e9
     del src, dst
     cpu = None
-    if not args.no_cpu:
+    if with_cpu:
         from oracle import lapack
         if lapack.available():
             lapack.set_num_threads(1)
@@ -348,6 +351,30 @@ def bench_getrf32(args) -> int:
                    "kind": "reference",
                    "sample": (f"dgetrf_ of {sample} matrices of the same batch through host "
                               f"LAPACK ({lapack.load()[2]}), one thread")}
+    del A0
+    handle.close()
+    return t, gbs, copy_gbs, nonzero, cpu
+
+
+def getrf32_roofline(t, gbs, copy_gbs):
+    n = GETRF32_N
+    per = 2 * n * n * 8 + n * 4 + 4
+    return {"kernel": "getrf_small_kernel<double,32,32,pivot> (team of 32 lanes per matrix)",
+            "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
+            "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
+            "traffic": measured_traffic("getrf_small_kernel<double,32,32,pivot>",
+                                        "traffic_getrf_n32_batch131072.md"),
+            "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4),
+            "device_copy_gbs": round(copy_gbs, 1),
+            "frac_of_device_copy": round(gbs / copy_gbs, 4)}
+
+
+def bench_getrf32(args) -> int:
+    """--workload getrf32: the n=32 dgetrf HBM roofline as its own JSON line."""
+    import torch
+    torch.cuda.set_device(0)
+    n, bc = GETRF32_N, GETRF32_BATCH
+    t, gbs, copy_gbs, nonzero, cpu = measure_getrf32(args, not args.no_cpu)
     out = {
         "metric": "n=32 dgetrf_strided_batched HBM roofline fraction",
         "value": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
@@ -358,19 +385,263 @@ def bench_getrf32(args) -> int:
         "config": {"workload": "plain FP64 dgetrf_strided_batched, n=32, lda=32, stride 1024",
                    "n": n, "batch": bc},
         "matrices_per_s": round(bc / t, 1),
-        "nonzero_info": int((info != 0).sum()),
-        "roofline": {"kernel": "getrf_small_kernel<double,32,32,pivot> (team of 32 lanes per matrix)",
-                     "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
-                     "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
-                     "traffic": measured_traffic("getrf_small_kernel<double,32,32,pivot>",
-                                                 "traffic_getrf_n32_batch131072.md"),
-                     "bytes_per_matrix": per, "launch_ms": round(t * 1e3, 4),
-                     "device_copy_gbs": round(copy_gbs, 1),
-                     "frac_of_device_copy": round(gbs / copy_gbs, 4)},
+        "nonzero_info": nonzero,
+        "roofline": getrf32_roofline(t, gbs, copy_gbs),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
     return 0
+
+
+def event_ms(torch, fn, reps: int, pre=None) -> float:
+    """Mean duration (ms) of fn() over reps calls, HIP events on the current
+    stream; pre() (restoring inputs) runs before each start event."""
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    tot = 0.0
+    for _ in range(reps):
+        if pre:
+            pre()
+        e0.record(s)
+        fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        tot += e0.elapsed_time(e1)
+    return tot / reps
+
+
+def measure_config1(torch, ra, handle, reps: int):
+    """C1: plain FP64 dgetrf + dgetrs (1 RHS) of 1024 matrices of order 32,
+    U(-1,1), as captured-graph replays.  The configured batch is 17 MB, a few
+    microseconds of HBM time, below launch latency: one graph holds 16 factor
+    + solve pairs, each on its own pristine copy of the batch (restored
+    outside the timed region), and a replay is timed with HIP events."""
+    n, bc, K = 32, 1024, 16
+    g = torch.Generator(device="cuda").manual_seed(20261203)
+    A0 = torch.rand((bc, n, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    B0 = torch.rand((bc, 1, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    As = [A0.clone() for _ in range(K)]
+    Bs = [B0.clone() for _ in range(K)]
+    ipiv = torch.zeros((K, bc, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros((K, bc), dtype=torch.int32, device="cuda")
+
+    def pair(k):
+        ra.check("rocsolver_dgetrf_strided_batched", handle.value, n, n, As[k].data_ptr(), n,
+                 n * n, ipiv[k].data_ptr(), n, info[k].data_ptr(), bc)
+        ra.check("rocsolver_dgetrs_strided_batched", handle.value, ra.OP_NONE, n, 1,
+                 As[k].data_ptr(), n, n * n, ipiv[k].data_ptr(), n, Bs[k].data_ptr(), n, n, bc)
+
+    def restore():
+        for k in range(K):
+            As[k].copy_(A0)
+            Bs[k].copy_(B0)
+    pair(0)
+    torch.cuda.synchronize()
+    cs = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    restore()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=cs):
+        handle.set_stream(cs)
+        for k in range(K):
+            pair(k)
+    handle.set_stream(torch.cuda.current_stream())
+    ms = event_ms(torch, graph.replay, reps, restore) / K
+    # eager pairs for comparison (launch overhead included)
+    eager = event_ms(torch, lambda: pair(0), reps, restore)
+    X = Bs[0][:, 0, :].double()
+    A = A0.transpose(1, 2)
+    R = (torch.matmul(A, X.unsqueeze(2)).squeeze(2) - B0[:, 0, :]).abs().amax(dim=1)
+    berr = (R / (A.abs().sum(dim=2).amax(dim=1) * X.abs().amax(dim=1))).amax().item()
+    per_getrf, per_getrs = 2 * n * n * 8 + n * 4 + 4, n * n * 8 + n * 4 + 2 * n * 8
+    gbs = bc * (per_getrf + per_getrs) / (ms * 1e-3) / 1e9
+    return {"workload": "dgetrf + dgetrs, n=32, batch 1024, 1 RHS, U(-1,1)",
+            "ms_per_factor_and_solve_graph": round(ms, 5), "ms_eager": round(eager, 4),
+            "systems_per_s": round(bc / (ms * 1e-3), 1),
+            "max_backward_error": berr,
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_TBS * 1e3,
+                         "unit": "GB/s", "frac": round(gbs / 1e3 / HBM_PEAK_TBS, 4),
+                         "bytes_per_system": per_getrf + per_getrs,
+                         "note": "17 MB per call: MALL-resident, launch-latency bound"}}
+
+
+def measure_config2(torch, ra, handle, reps: int):
+    """C2: plain FP64 dgetrf of 16384 matrices of order 128 (N(0,1) + 12.8 I)
+    and dgetrs with 4 N(0,1) right-hand sides; A restored outside the timed
+    region before each factorisation."""
+    n, bc, nrhs = 128, 16384, 4
+    g = torch.Generator(device="cuda").manual_seed(20261204)
+    A0 = torch.randn((bc, n, n), dtype=torch.float64, device="cuda", generator=g)
+    A0.diagonal(dim1=1, dim2=2).add_(0.1 * n)
+    B0 = torch.randn((bc, nrhs, n), dtype=torch.float64, device="cuda", generator=g)
+    A, B = torch.empty_like(A0), torch.empty_like(B0)
+    ipiv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+
+    def getrf():
+        ra.check("rocsolver_dgetrf_strided_batched", handle.value, n, n, A.data_ptr(), n, n * n,
+                 ipiv.data_ptr(), n, info.data_ptr(), bc)
+
+    def getrs():
+        ra.check("rocsolver_dgetrs_strided_batched", handle.value, ra.OP_NONE, n, nrhs,
+                 A.data_ptr(), n, n * n, ipiv.data_ptr(), n, B.data_ptr(), n, n * nrhs, bc)
+    A.copy_(A0)
+    getrf()
+    tf = event_ms(torch, getrf, reps, lambda: A.copy_(A0))
+    ts = event_ms(torch, getrs, reps, lambda: B.copy_(B0))
+    fl_f, fl_s = lu_flops(n), nrhs * (2.0 * n * n - n)
+    by_f, by_s = 2 * n * n * 8 + n * 4 + 4, n * n * 8 + n * 4 + 2 * nrhs * n * 8
+    del A0, A
+    return {"workload": "dgetrf + dgetrs, n=128, batch 16384, 4 RHS, N(0,1) + 12.8 I",
+            "dgetrf_ms": round(tf, 4), "dgetrs_ms": round(ts, 4),
+            "dgetrf_gbs": round(bc * by_f / (tf * 1e-3) / 1e9, 1),
+            "dgetrf_tflops": round(bc * fl_f / (tf * 1e-3) / 1e12, 3),
+            "dgetrs_gbs": round(bc * by_s / (ts * 1e-3) / 1e9, 1),
+            "nonzero_info": int((info != 0).sum()),
+            "roofline": {"kernel": "lu_fused_kernel<double,double,128>", "bound": "hbm",
+                         "achieved": round(bc * by_f / (tf * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_TBS * 1e3, "unit": "GB/s",
+                         "frac": round(bc * by_f / (tf * 1e-3) / 1e12 / HBM_PEAK_TBS, 4),
+                         "fp64_frac": round(bc * fl_f / (tf * 1e-3) / 1e12 / 78.6, 4),
+                         "bytes_per_matrix": by_f, "flops_per_matrix": fl_f}}
+
+
+def measure_config4(torch, ra, handle, reps: int):
+    """C4: variable-size FP64 gesv of 10000 matrices (n log-uniform integers in
+    [8, 512], U(-1,1) + n I, 1..4 RHS) in one persistent launch, replayed from a
+    captured graph; inputs restored outside the timed region."""
+    import numpy as np
+
+    from rocsolver_amd import workloads as wl
+    n, nrhs = wl.config4_sizes(10000)
+    vb = ra.VBatch(n, nrhs, torch.float64)
+    gen = torch.Generator(device="cuda").manual_seed(wl.BASE_SEED + 4)
+    vb.A.uniform_(-1.0, 1.0, generator=gen)
+    vb.B.uniform_(-1.0, 1.0, generator=gen)
+    diag = np.concatenate([vb.offA[b] + np.arange(n[b]) * (n[b] + 1) for b in range(len(n))])
+    vb.A[torch.from_numpy(diag).cuda()] += torch.from_numpy(np.repeat(n.astype(np.float64), n)).cuda()
+    A0, B0 = vb.A.clone(), vb.B.clone()
+
+    def restore():
+        vb.A.copy_(A0)
+        vb.B.copy_(B0)
+    restore()
+    vb.gesv(handle)
+    torch.cuda.synchronize()
+    cs = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    restore()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=cs):
+        handle.set_stream(cs)
+        vb.gesv(handle)
+    handle.set_stream(torch.cuda.current_stream())
+    ms = event_ms(torch, graph.replay, reps, restore)
+    flops = float(sum(gesv_flops(int(k), int(r)) for k, r in zip(n, nrhs)))
+    return {"workload": "variable-size dgesv, 10000 matrices, n log-uniform in [8, 512], 1-4 RHS",
+            "graph_replay_ms": round(ms, 4), "matrices_per_s": round(len(n) / (ms * 1e-3), 1),
+            "tflops": round(flops / (ms * 1e-3) / 1e12, 3),
+            "fp64_frac": round(flops / (ms * 1e-3) / 1e12 / 78.6, 4),
+            "nonzero_info": int((vb.info != 0).sum())}
+
+
+def measure_config5(torch, ra, handle, reps: int):
+    """C5: adjoint of X = A^-1 B, FP64, 2048 systems of order 96 (N(0,1) + 10 I),
+    16 RHS: forward gesv, then the backward dB = A^-T dX (transposed solve
+    with the stored LU and pivots) and dA = -dB X^T."""
+    n, bc, nrhs = 96, 2048, 16
+    g = torch.Generator(device="cuda").manual_seed(20261205)
+    A0 = torch.randn((bc, n, n), dtype=torch.float64, device="cuda", generator=g)
+    A0.diagonal(dim1=1, dim2=2).add_(10.0)
+    B0 = torch.randn((bc, nrhs, n), dtype=torch.float64, device="cuda", generator=g)
+    dX = torch.randn((bc, nrhs, n), dtype=torch.float64, device="cuda", generator=g)
+    LU, X = torch.empty_like(A0), torch.empty_like(B0)
+    dB, dA = torch.empty_like(B0), torch.empty_like(A0)
+    ipiv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
+    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+
+    def fwd():
+        ra.check("rocsolver_dgesv_strided_batched", handle.value, n, nrhs, LU.data_ptr(), n, n * n,
+                 ipiv.data_ptr(), n, X.data_ptr(), n, n * nrhs, info.data_ptr(), bc)
+
+    def bwd():
+        ra.check("rocsolver_amd_dgesv_adjoint_strided_batched", handle.value, n, nrhs,
+                 LU.data_ptr(), n, n * n, ipiv.data_ptr(), n, X.data_ptr(), n, n * nrhs,
+                 dX.data_ptr(), n, n * nrhs, dB.data_ptr(), n, n * nrhs, dA.data_ptr(), n, n * n,
+                 bc)
+
+    def restore():
+        LU.copy_(A0)
+        X.copy_(B0)
+    tf = event_ms(torch, fwd, reps, restore)
+    restore()
+    fwd()
+    tb = event_ms(torch, bwd, reps)
+    per = 184704  # LU read + X, dX read + dB, dA written (bytes per system)
+    return {"workload": "adjoint of X = A^-1 B, FP64, n=96, batch 2048, 16 RHS",
+            "forward_gesv_ms": round(tf, 4), "backward_adjoint_ms": round(tb, 4),
+            "roofline": {"kernel": "adjoint (transposed getrs + outer product)", "bound": "hbm",
+                         "achieved": round(bc * per / (tb * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_TBS * 1e3, "unit": "GB/s",
+                         "frac": round(bc * per / (tb * 1e-3) / 1e12 / HBM_PEAK_TBS, 4),
+                         "bytes_per_system": per}}
+
+
+def measure_half_width(torch, ra, handle, A, B, reps: int):
+    """The metric workload solved with equilibrated FP16 / BF16 matrix-core
+    operands in the FP32 LU's updates (rocsolver_amd_dxgesv_strided_batched):
+    time per batch, iteration histogram, FP64 fallbacks, backward error."""
+    out = {}
+    bc, n, lda = A.shape
+    _, nrhs, ldb = B.shape
+    X = torch.zeros((bc, nrhs, n), dtype=torch.float64, device="cuda")
+    it = torch.zeros(bc, dtype=torch.int32, device="cuda")
+    info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+    for name in ("fp16", "bf16"):
+        def run():
+            ra.check("rocsolver_amd_dxgesv_strided_batched", handle.value, ra.FACTOR[name], n, nrhs,
+                     A.data_ptr(), lda, n * lda, B.data_ptr(), ldb, nrhs * ldb, X.data_ptr(), n,
+                     nrhs * n, it.data_ptr(), info.data_ptr(), None, bc)
+        run()
+        ms = event_ms(torch, run, reps)
+        Arm = A.transpose(1, 2)
+        R = B.transpose(1, 2) - torch.matmul(Arm, X.transpose(1, 2))
+        bwd = (R.abs().amax(dim=1) / (Arm.abs().sum(dim=2).amax(dim=1)[:, None]
+                                      * X.abs().amax(dim=2) + B.abs().amax(dim=2))).amax().item()
+        itc = it.cpu()
+        hist = {int(k): int(v) for k, v in zip(*torch.unique(itc, return_counts=True))}
+        out[name] = {"ms": round(ms, 4), "solves_per_s": round(bc / (ms * 1e-3), 1),
+                     "iter_histogram": hist, "fallback_matrices": int((itc < 0).sum()),
+                     "max_backward_error": bwd, "target_10_n_eps64": 10 * n * EPS64}
+    return out
+
+
+def other_configs(args, torch, ra, A, B):
+    """Every other configuration, measured in the same run (each guarded: a
+    failure is reported, not hidden)."""
+    handle = ra.Handle()
+    reps = max(3, min(args.steps, 10))
+    res = {}
+    for key, fn in (("c1_getrf_getrs_n32_b1024", lambda: measure_config1(torch, ra, handle, reps)),
+                    ("c2_getrf_getrs_n128_b16384", lambda: measure_config2(torch, ra, handle, reps)),
+                    ("c4_vbatched_gesv_10000", lambda: measure_config4(torch, ra, handle, reps)),
+                    ("c5_adjoint_n96_b2048", lambda: measure_config5(torch, ra, handle, reps)),
+                    ("c3_half_width_factors",
+                     lambda: measure_half_width(torch, ra, handle, A, B, reps))):
+        try:
+            res[key] = fn()
+        except Exception as e:  # report, do not hide
+            res[key] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.synchronize()
+    try:
+        t, gbs, copy_gbs, nonzero, _ = measure_getrf32(args, False)
+        res["getrf_n32_b131072_hbm"] = dict(getrf32_roofline(t, gbs, copy_gbs),
+                                            nonzero_info=nonzero)
+    except Exception as e:
+        res["getrf_n32_b131072_hbm"] = {"error": f"{type(e).__name__}: {e}"}
+    handle.close()
+    return res
 
 
 def main() -> int:
@@ -381,6 +652,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=8.0,
                     help="seconds of single-thread host LAPACK in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the other configurations (C1, C2, C4, C5, half-width factors, "
+                         "n=32 HBM roofline) that the default line also reports")
     ap.add_argument("--workload", choices=["mixed", "vbatched", "getrf32"], default="mixed",
                     help="mixed: the headline n=256 mixed-precision solve (default); vbatched: "
                          "10000 matrices of orders 8..512 in one persistent launch; getrf32: "
@@ -475,6 +749,7 @@ def main() -> int:
     del LU
 
     cpu = None if args.no_cpu else cpu_baseline(torch, A, B, X, args.cpu_budget)
+    others = None if args.no_others else other_configs(args, torch, ra, A, B)
 
     out = {
         "metric": "solves/s n=256 batch 4096 8 RHS: FP32-MFMA+IR vs plain FP64 dgesv at eps64",
@@ -510,7 +785,10 @@ def main() -> int:
             "launch_ms": round(lu_t * 1e3, 4),
         },
         "cpu_baseline": cpu,
+        "other_configs": others,
     }
+    if others and "getrf_n32_b131072_hbm" in others and "frac" in others["getrf_n32_b131072_hbm"]:
+        out["getrf_n32_hbm_fraction"] = others["getrf_n32_b131072_hbm"]["frac"]
     print(json.dumps(out))
     return 0
 
