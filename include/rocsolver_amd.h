@@ -404,11 +404,13 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
  *   berr   device or NULL: per matrix max_c ||R_c||_inf / (||A||_inf ||X_c||_inf)
  *          at convergence (-1 where the FP64 path produced X)
  * Workspace comes from the handle (size query supported): about
- * bc * (8 n^2 + 4 n) bytes.  Supported: 64 < n <= 512 and
+ * bc * (8 n^2 + 4 n) bytes.  Supported: n <= 512 and
  * lda * n * 8 < 2^31 (rocblas_status_not_implemented otherwise).  With
  * nrhs > 8 (> 4 when n > 256) the columns are refined in groups of 8 (4):
- * each group iterates until its own columns pass the test, and iter is the
- * largest group count. */
+ * each group iterates until its own columns pass the test (its solution is
+ * the one a call with those columns alone gives), and iter is the largest
+ * group count; DSGESV instead refines every column until all pass, so a
+ * group that passed early gets no further corrections here. */
 rocblas_status rocsolver_amd_dsgesv_strided_batched(
     rocblas_handle handle, const rocblas_int n, const rocblas_int nrhs, const double* A,
     const rocblas_int lda, const rocblas_stride strideA, const double* B, const rocblas_int ldb,

@@ -309,7 +309,7 @@ rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, c
         rocblas_status st = reset_info(stream, info, bc);
         return st != rocblas_status_success ? st : reset_info(stream, iter, bc);
     }
-    if(n <= SMALL_MAX || n > FUSED_MAX || !fits_buffer<double>(lda, n)
+    if(n > FUSED_MAX || !fits_buffer<double>(lda, n)
        || !fits_buffer<double>(ldb, nrhs) || !fits_buffer<double>(ldx, nrhs))
         return rocblas_status_not_implemented;
     rocblas_device_malloc mem(handle, ws);

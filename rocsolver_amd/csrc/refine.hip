@@ -37,7 +37,9 @@
 
 #include <cfloat>
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <utility>
 
 namespace rsa
 {
@@ -566,20 +568,29 @@ static void launch_ir(hipStream_t stream, int n, int nrhs, Batch<double> A, int 
 // Batches from this size on are solved in two halves on two streams.
 constexpr int SPLIT_MIN = 256;
 
-// One internal stream per device (default priority, non-blocking), created
-// on first use and kept for the life of the process; nullptr if it cannot be
-// created (the solve then runs on the call stream alone).
-static hipStream_t aux_stream()
+// One internal stream per caller stream (default priority, non-blocking),
+// created on first use and kept for the life of the process; nullptr if it
+// cannot be created (the solve then runs on the call stream alone).  Keyed
+// by the caller's stream, so calls on different streams -- different
+// handles, threads, or a graph being captured on one of them -- never share
+// an internal stream; calls on one stream are ordered by that stream anyway.
+static hipStream_t aux_stream(hipStream_t caller)
 {
     static std::mutex m;
-    static hipStream_t streams[64] = {};
+    static std::map<std::pair<int, hipStream_t>, hipStream_t> streams;
     int dev = 0;
-    if(hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    if(hipGetDevice(&dev) != hipSuccess)
         return nullptr;
     std::lock_guard<std::mutex> lock(m);
-    if(!streams[dev] && hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking) != hipSuccess)
-        streams[dev] = nullptr;
-    return streams[dev];
+    const auto key = std::make_pair(dev, caller);
+    auto it = streams.find(key);
+    if(it != streams.end())
+        return it->second;
+    hipStream_t s = nullptr;
+    if(hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        s = nullptr;
+    streams[key] = s;
+    return s;
 }
 
 rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Batch<double> A,
@@ -588,7 +599,7 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
 {
     if(factor < 0 || factor > 2)
         return rocblas_status_invalid_value;
-    if(n <= SMALL_MAX || n > FUSED_MAX)
+    if(n > FUSED_MAX)
         return rocblas_status_not_implemented;
     if(bc <= 0 || nrhs <= 0)
         return rocblas_status_success;
@@ -681,7 +692,7 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
                                lda, rs, cs, int64_t(n));
         RSA_LAUNCH_CHECK();
     }
-    hipStream_t aux = bc >= SPLIT_MIN ? aux_stream() : nullptr;
+    hipStream_t aux = bc >= SPLIT_MIN ? aux_stream(stream) : nullptr;
     if(aux)
     {
         // two halves at once, the second on an internal stream: each half's

@@ -23,7 +23,8 @@ def dev(torch, a):
 
 # batches of 256 and more are solved in two staggered halves on two streams
 @pytest.mark.parametrize("n,nrhs,bc", [(96, 16, 6), (128, 4, 5), (256, 8, 6), (200, 1, 4), (512, 2, 3),
-                                       (70, 3, 3), (96, 3, 301)])
+                                       (70, 3, 3), (96, 3, 301), (1, 1, 4), (5, 2, 3), (17, 3, 4),
+                                       (32, 8, 6), (64, 4, 5)])
 def test_dsgesv_against_oracle(torch_gpu, handle, n, nrhs, bc):
     print(f"dsgesv n={n} nrhs={nrhs} bc={bc}", flush=True)
     A, B, kappa = wl.config3(bc=bc, n=n, nrhs=nrhs)
@@ -74,7 +75,8 @@ def test_dsgesv_fallback_paths(torch_gpu, handle):
 
 
 @pytest.mark.parametrize("factor", ["fp16", "bf16"])
-@pytest.mark.parametrize("n,nrhs,bc", [(96, 4, 6), (256, 8, 4), (300, 3, 2), (80, 2, 260)])
+@pytest.mark.parametrize("n,nrhs,bc", [(96, 4, 6), (256, 8, 4), (300, 3, 2), (80, 2, 260),
+                                       (40, 2, 4)])
 def test_half_width_factor_refines_to_fp64(torch_gpu, handle, factor, n, nrhs, bc):
     """Equilibrated FP16/BF16-operand LU + FP64 refinement: well-conditioned
     systems converge by refinement to FP64 backward error; the result agrees
@@ -225,3 +227,104 @@ def test_dsgesv_metric_batch(torch_gpu, handle):
         assert fe <= kappa[b].item() * n * EPS64 * 10
     print(f"iterations: {np.bincount(itc[itc >= 0]).tolist()}, fallbacks {(itc < 0).sum()}, "
           f"max backward error {float(be.max()):.2e}", flush=True)
+
+
+def test_dsgesv_capture_and_eager_threads(torch_gpu):
+    """One thread captures a two-stream mixed solve in a graph while another
+    thread runs an eager two-stream solve on its own handle and stream: the
+    library's internal streams are per caller stream, so the eager call runs
+    (and is correct) and the graph replays the captured call exactly."""
+    import threading
+    torch = torch_gpu
+    n, nrhs, bc = 96, 2, 300
+    A1, B1, _ = wl.config3(bc=bc, n=n, nrhs=nrhs, seed=41)
+    A2, B2, _ = wl.config3(bc=bc, n=n, nrhs=nrhs, seed=42)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    ha, hb = ra.Handle(sa), ra.Handle(sb)
+    try:
+        bufs = []
+        for A, B in ((A1, B1), (A2, B2)):
+            bufs.append((dev(torch, A), dev(torch, B),
+                         torch.zeros((bc, nrhs, n), dtype=torch.float64, device="cuda"),
+                         torch.zeros(bc, dtype=torch.int32, device="cuda"),
+                         torch.zeros(bc, dtype=torch.int32, device="cuda")))
+
+        def call(h, k):
+            Ad, Bd, X, it, info = bufs[k]
+            return ra.call("rocsolver_amd_dsgesv_strided_batched", h.value, n, nrhs, Ad.data_ptr(),
+                           n, n * n, Bd.data_ptr(), n, n * nrhs, X.data_ptr(), n, n * nrhs,
+                           it.data_ptr(), info.data_ptr(), 0, bc)
+        # eager references (also size both handles' workspaces)
+        with torch.cuda.stream(sa):
+            assert call(ha, 0) == 0
+        with torch.cuda.stream(sb):
+            assert call(hb, 1) == 0
+        torch.cuda.synchronize()
+        ref = [(b[2].clone(), b[3].clone()) for b in bufs]
+        for b in bufs:
+            b[2].zero_()
+        torch.cuda.synchronize()
+        inside = threading.Barrier(2)
+        status = {}
+
+        def capturer():
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=sa, capture_error_mode="thread_local"):
+                status["a"] = call(ha, 0)
+                inside.wait()  # the other thread runs while this capture is open
+                inside.wait()
+            status["graph"] = g
+
+        def eager():
+            inside.wait()
+            with torch.cuda.stream(sb):
+                status["b"] = call(hb, 1)
+            sb.synchronize()
+            inside.wait()
+        ta, tb = threading.Thread(target=capturer), threading.Thread(target=eager)
+        ta.start()
+        tb.start()
+        ta.join(60)
+        tb.join(60)
+        assert status.get("a") == 0 and status.get("b") == 0, status
+        torch.cuda.synchronize()
+        assert torch.equal(bufs[1][2], ref[1][0]) and torch.equal(bufs[1][3], ref[1][1])
+        assert not torch.equal(bufs[0][2], ref[0][0])  # captured, not run
+        status["graph"].replay()
+        torch.cuda.synchronize()
+        assert torch.equal(bufs[0][2], ref[0][0]) and torch.equal(bufs[0][3], ref[0][1])
+        assert torch.equal(bufs[1][2], ref[1][0])  # the eager result was not touched
+    finally:
+        ha.close()
+        hb.close()
+
+
+@pytest.mark.parametrize("nrhs", [16, 12])
+def test_dsgesv_wide_rhs_iter(torch_gpu, handle, nrhs):
+    """More right-hand sides than one refinement group (8 columns): every
+    group refines until its own columns pass LAPACK's stopping test, and ITER
+    is the largest group count.  Pinned exactly: the solution of each group
+    equals, bit for bit, that of a call with the group's columns alone, and
+    ITER is the maximum of those calls' ITER.  Against the oracle (LAPACK
+    semantics: all columns refined together) the outcome class matches and
+    ITER stays within the FP32-rounding margin of the other tests."""
+    torch = torch_gpu
+    n, bc = 128, 24
+    A, B, kappa = wl.config3(bc=bc, n=n, nrhs=nrhs, seed=77)
+    Ad = dev(torch, A)
+    X, it, info, berr = ra.dsgesv(handle, Ad, dev(torch, B))
+    torch.cuda.synchronize()
+    it, X = it.cpu().numpy(), X.cpu().numpy()
+    its = []
+    for c0 in range(0, nrhs, 8):
+        Bg = np.ascontiguousarray(B[:, c0:c0 + 8, :])
+        Xg, itg, _, _ = ra.dsgesv(handle, Ad, dev(torch, Bg))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(X[:, c0:c0 + 8, :], Xg.cpu().numpy())
+        its.append(itg.cpu().numpy())
+    np.testing.assert_array_equal(it, np.max(its, axis=0))
+    Xo, it_o, info_o = oracle.dsgesv(A, n, B)
+    assert np.all((it >= 0) == (it_o >= 0))
+    assert np.all(np.abs(it - it_o) <= 2)
+    be = oracle.solve_backward_error(A, n, X, B)
+    assert be.max() <= 10 * n * EPS64
