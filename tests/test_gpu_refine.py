@@ -328,3 +328,31 @@ def test_dsgesv_wide_rhs_iter(torch_gpu, handle, nrhs):
     assert np.all(np.abs(it - it_o) <= 2)
     be = oracle.solve_backward_error(A, n, X, B)
     assert be.max() <= 10 * n * EPS64
+
+
+@pytest.mark.parametrize("factor", ["fp16", "bf16"])
+def test_half_width_factors_on_metric_workload(torch_gpu, handle, factor):
+    """The metric workload's matrices (A = U diag(s) V^T, kappa log-uniform in
+    [1e2, 1e6], 8 RHS) with FP16 / BF16 factors: per matrix the outcome
+    (refined vs FP64 fallback) is reported against the oracle's dsgesv (FP32
+    factors): a system refined with half-width factors is one the FP32 path
+    also refines; every system reaches FP64 backward error <= 10 n eps64 and
+    the oracle's solution within 10 kappa n eps64, refined or not."""
+    torch = torch_gpu
+    n, nrhs, bc = 256, 8, 48
+    A, B, kappa = wl.config3(bc=bc, n=n, nrhs=nrhs, seed=2026)
+    Xo, it_o, info_o = oracle.dsgesv(A, n, B)
+    X, it, info, berr = ra.dsgesv(handle, dev(torch, A), dev(torch, B), factor=factor)
+    torch.cuda.synchronize()
+    X, it, info = X.cpu().numpy(), it.cpu().numpy(), info.cpu().numpy()
+    refined = it >= 0
+    print(f"{factor}: refined {refined.sum()} / {bc} (oracle FP32: {(it_o >= 0).sum()}), "
+          f"fallbacks {(it < 0).sum()}, iterations {sorted(set(it[refined].tolist()))}", flush=True)
+    assert np.all(it_o[refined] >= 0)
+    assert np.all((it < 0) <= (it == -31))  # the only fallback cause here: no convergence
+    np.testing.assert_array_equal(info, info_o)
+    be = oracle.solve_backward_error(A, n, X, B)
+    assert be.max() <= 10 * n * EPS64
+    for b in range(bc):
+        fe = np.abs(X[b] - Xo[b]).max() / np.abs(Xo[b]).max()
+        assert fe <= 10 * kappa[b] * n * EPS64
