@@ -19,10 +19,10 @@ OTHER = "void at::native::fill_kernel(int)"
 
 def _write(path, rows):
     with open(path, "w", newline="") as f:
-        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Counter_Value"])
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Grid_Size", "Counter_Value"])
         w.writeheader()
-        for name, kib in rows:
-            w.writerow({"Kernel_Name": name, "Counter_Value": kib})
+        for name, grid, kib in rows:
+            w.writerow({"Kernel_Name": name, "Grid_Size": grid, "Counter_Value": kib})
 
 
 def test_parse_scales_by_the_copy(tmp_path):
@@ -30,16 +30,23 @@ def test_parse_scales_by_the_copy(tmp_path):
     # (scale 2); WRITE_SIZE reports it whole (scale 1)
     gib_kib = (1 << 30) / 1024
     fetch, write = tmp_path / "f.csv", tmp_path / "w.csv"
-    _write(fetch, [(COPY, gib_kib / 2), (LU, 1000.0), (LU, 3000.0), (OTHER, 5.0)])
-    _write(write, [(COPY, gib_kib), (LU, 4000.0), (LU, 4000.0), (OTHER, 5.0)])
+    _write(fetch, [(COPY, 1 << 27, gib_kib / 2), (LU, 4096, 1000.0), (LU, 4096, 3000.0),
+                   (LU, 2048, 700.0), (OTHER, 64, 5.0)])
+    _write(write, [(COPY, 1 << 27, gib_kib), (LU, 4096, 4000.0), (LU, 4096, 4000.0),
+                   (LU, 2048, 900.0), (OTHER, 64, 5.0)])
     res = traffic.parse(str(fetch), str(write))
     assert res["copy_kernel"] == COPY
     assert abs(res["fetch_scale"] - 2.0) < 1e-12 and abs(res["write_scale"] - 1.0) < 1e-12
-    assert list(res["kernels"]) == [LU]  # only this library's kernels are reported
-    k = res["kernels"][LU]
-    assert k["dispatches"] == 2
-    assert k["fetch_bytes"] == 2000.0 * 1024 * 2  # mean per dispatch, KiB -> bytes, scaled
-    assert k["write_bytes"] == 4000.0 * 1024
+    # only this library's kernels, one entry per grid size (a launch over part
+    # of the batch is not averaged with one over all of it)
+    ks = res["kernels"]
+    assert [(k["kernel"], k["grid"]) for k in ks] == [
+        ("void rsa::getrf_small_kernel<double, 32, 32, true, true>", 2048),
+        ("void rsa::getrf_small_kernel<double, 32, 32, true, true>", 4096)]
+    assert ks[0]["dispatches"] == 1 and ks[1]["dispatches"] == 2
+    assert ks[1]["fetch_bytes"] == 2000.0 * 1024 * 2  # mean per dispatch, KiB -> bytes, scaled
+    assert ks[1]["write_bytes"] == 4000.0 * 1024
+    assert ks[0]["fetch_bytes"] == 700.0 * 1024 * 2 and ks[0]["write_bytes"] == 900.0 * 1024
 
 
 def test_committed_summaries_are_read():
