@@ -3,7 +3,8 @@
 // Each *_impl reproduces the reference impl's control flow:
 //   null handle -> invalid_handle;  argCheck in the reference order
 //   (values, then sizes, then -- unless in device-memory size-query mode --
-//   pointers);  size query -> report the workspace (none is needed here);
+//   pointers);  size query -> report the workspace (the fused LU keeps its
+//   inverted diagonal blocks there; the other paths need none);
 //   quick returns;  then the work, queued on the handle's stream.
 // References: roclapack_getrf_strided_batched.cpp:7-84 (getrf impl),
 // roclapack_getf2.hpp:488-517 (getf2/getrf argCheck), roclapack_getrs.hpp:17-49
@@ -41,8 +42,13 @@ rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool 
     const int64_t mn = int64_t(m) * n;
     if(!query && ((mn && !hasA) || (mn && pivot && !ipiv) || (bc && !info)))
         return rocblas_status_invalid_pointer;
+    // the fused path keeps the inverted diagonal blocks of L in workspace
+    const bool fused = pivot && m == n && n > SMALL_MAX && n <= FUSED_MAX
+                       && fits_buffer<T>(lda, n);
+    const size_t ws = fused && bc > 0 ? lu_fused_workspace_bytes<T>(n, bc) : 0;
     if(query)
-        return report_no_workspace(handle);
+        return ws ? rocblas_set_optimal_device_memory_size(handle, ws)
+                  : report_no_workspace(handle);
 
     hipStream_t stream;
     rocblas_get_stream(handle, &stream);
@@ -55,10 +61,15 @@ rocblas_status getrf_impl(rocblas_handle handle, int m, int n, Batch<T> A, bool 
         RSA_LOG_STAGE(handle, "getrf_small", "m", m, "n", n, "pivot", int(pivot), "bc", bc);
         return getrf_small<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
     }
-    if(pivot && m == n && n <= FUSED_MAX && fits_buffer<T>(lda, n))
+    if(fused)
     {
         RSA_LOG_STAGE(handle, "getrf_fused", "n", n, "bc", bc);
-        return lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
+        rocblas_device_malloc mem(handle, ws);
+        if(!mem)
+            return rocblas_status_memory_error;
+        return lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc,
+                              strided(static_cast<T*>(static_cast<void*>(mem)),
+                                      lu_fused_inv_elems(n)));
     }
     RSA_LOG_STAGE(handle, "getrf_blocked", "m", m, "n", n, "pivot", int(pivot), "bc", bc);
     return getrf_blocked<T>(stream, m, n, A, lda, ipiv, strideP, info, bc, pivot);
@@ -125,8 +136,12 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
     if(!query
        && ((n && !hasA) || (n && !ipiv) || (int64_t(nrhs) * n && !hasB) || (bc && !info)))
         return rocblas_status_invalid_pointer;
+    const bool fused = n > SMALL_MAX && n <= FUSED_MAX && fits_buffer<T>(lda, n)
+                       && fits_buffer<T>(ldb, nrhs);
+    const size_t ws = fused && nrhs > 0 && bc > 0 ? lu_fused_workspace_bytes<T>(n, bc) : 0;
     if(query)
-        return report_no_workspace(handle);
+        return ws ? rocblas_set_optimal_device_memory_size(handle, ws)
+                  : report_no_workspace(handle);
 
     if(bc == 0)
         return rocblas_status_success;
@@ -136,9 +151,15 @@ rocblas_status gesv_impl(rocblas_handle handle, int n, int nrhs, Batch<T> A, boo
         return reset_info(stream, info, bc);
     if(n <= SMALL_MAX)
         return gesv_small<T>(stream, n, nrhs, A, lda, ipiv, strideP, B, ldb, info, bc);
-    if(n <= FUSED_MAX && fits_buffer<T>(lda, n) && fits_buffer<T>(ldb, nrhs))
+    if(fused)
     {
-        rocblas_status st = lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc);
+        rocblas_device_malloc mem(handle, ws);
+        if(!mem)
+            return rocblas_status_memory_error;
+        rocblas_status st
+            = lu_fused<T, T>(stream, n, A, lda, A, lda, ipiv, strideP, info, bc,
+                             strided(static_cast<T*>(static_cast<void*>(mem)),
+                                     lu_fused_inv_elems(n)));
         if(st != rocblas_status_success)
             return st;
         return getrs_fused<T, T>(stream, false, n, nrhs, A, lda, ipiv, strideP, B, ldb, B, ldb,
@@ -345,8 +366,12 @@ rocblas_status dsgetrf_impl(rocblas_handle handle, int n, const double* A, int l
     const int64_t nn = int64_t(n) * n;
     if(!query && ((nn && (!A || !LU || !ipiv)) || (bc && !info)))
         return rocblas_status_invalid_pointer;
+    const bool fused = n > SMALL_MAX && n <= FUSED_MAX && fits_buffer<double>(lda, n)
+                       && fits_buffer<float>(ldlu, n);
+    const size_t ws = fused && bc > 0 ? lu_fused_workspace_bytes<float>(n, bc) : 0;
     if(query)
-        return report_no_workspace(handle);
+        return ws ? rocblas_set_optimal_device_memory_size(handle, ws)
+                  : report_no_workspace(handle);
     if(bc == 0)
         return rocblas_status_success;
     hipStream_t stream;
@@ -355,8 +380,15 @@ rocblas_status dsgetrf_impl(rocblas_handle handle, int n, const double* A, int l
         return reset_info(stream, info, bc);
     Batch<double> Ab = strided(const_cast<double*>(A), strideA);
     Batch<float> Lb = strided(LU, strideLU);
-    if(n > SMALL_MAX && n <= FUSED_MAX && fits_buffer<double>(lda, n) && fits_buffer<float>(ldlu, n))
-        return lu_fused<double, float>(stream, n, Ab, lda, Lb, ldlu, ipiv, strideP, info, bc);
+    if(fused)
+    {
+        rocblas_device_malloc mem(handle, ws);
+        if(!mem)
+            return rocblas_status_memory_error;
+        return lu_fused<double, float>(stream, n, Ab, lda, Lb, ldlu, ipiv, strideP, info, bc,
+                                       strided(static_cast<float*>(static_cast<void*>(mem)),
+                                               lu_fused_inv_elems(n)));
+    }
     // other orders: round to FP32 into LU, then the FP32 factorisation in place
     for(int64_t b0 = 0; b0 < bc; b0 += 65535)
     {

@@ -31,9 +31,21 @@ rocblas_status getrs_small(hipStream_t stream, bool trans, int n, int nrhs, Batc
 // when TI == TC).  Batch entries with skip[b] != 0 are not touched.
 constexpr int FUSED_MAX = 512;
 
+// WI: workspace for the inverted unit-lower diagonal blocks of the factor,
+// lu_fused_inv_elems(n) elements of TC per matrix (Batch addressing).
+inline int64_t lu_fused_inv_elems(int n)
+{
+    return int64_t((n + 31) / 32) * 1024;
+}
+template <typename TC>
+inline size_t lu_fused_workspace_bytes(int n, int bc)
+{
+    return sizeof(TC) * size_t(lu_fused_inv_elems(n)) * size_t(bc);
+}
+
 template <typename TI, typename TC>
 rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO, int ldo,
-                        int* ipiv, int64_t strideP, int* info, int bc,
+                        int* ipiv, int64_t strideP, int* info, int bc, Batch<TC> WI,
                         const int* skip = nullptr);
 
 // FP32 LU of diag(rs) A diag(cs) (A FP64; the scales of matrix b at
@@ -41,7 +53,8 @@ rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<
 // cores with FP16 (opk = 1) or BF16 (opk = 2) operands and FP32 accumulation.
 rocblas_status lu_fused_lowp(hipStream_t stream, int opk, int n, Batch<double> AI, int ldi,
                              Batch<float> AO, int ldo, int* ipiv, int64_t strideP, int* info,
-                             int bc, const double* rs, const double* cs, int64_t strideS);
+                             int bc, Batch<float> WI, const double* rs, const double* cs,
+                             int64_t strideS);
 
 // One-workgroup-per-system solve with the LU of getrf for SMALL_MAX < n <=
 // FUSED_MAX, arithmetic in TL; reads the right-hand sides from B (type TB),

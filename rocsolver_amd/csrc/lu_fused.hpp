@@ -7,7 +7,9 @@
 //      order (coalesced, staged through LDS) and placed in MFMA accumulators
 //      in the current logical row order;
 //   2. left-looking update, for every factored block column t < s:
-//        U_ts = L_tt^-1 C[t-rows]        (substitution, half-wave shuffles)
+//        U_ts = L_tt^-1 C[t-rows]        (inv(L_tt) C_t on the matrix cores;
+//                                         inv(L_tt) is formed once, right
+//                                         after panel t, into a workspace)
 //        C[rows below t] -= L21_t * U_ts (MFMA 16x16x4, L21_t staged in LDS in
 //                                         16-column stages prefetched one stage
 //                                         ahead into registers)
@@ -45,11 +47,16 @@ constexpr int FHC = 16; // columns per LDS round of the staging/transposition bu
 // selects the matrix-core operands of the trailing updates: the factor
 // precision itself, or FP16 / BF16 operands with FP32 accumulation (TC =
 // float; the panel, the block solves and the stored factors stay FP32).
-template <typename TI, typename TC, int NMAX, int OPK = MFMA_NATIVE>
+// INVW: the block solves U = L_tt^-1 C run as inv(L_tt) C on the matrix
+// cores, inv(L_tt) of every block column but the last kept in winv (32x32
+// column-major blocks, block t at winv + 1024 t); otherwise each is a
+// substitution with the solved values passed between lanes by v_readlane.
+template <typename TI, typename TC, int NMAX, int OPK = MFMA_NATIVE, bool INVW = false>
 __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, int ldi,
                                              TC* __restrict__ Ao, int ldo, int* __restrict__ ipiv,
                                              const double* __restrict__ rs = nullptr,
-                                             const double* __restrict__ cs = nullptr)
+                                             const double* __restrict__ cs = nullptr,
+                                             TC* __restrict__ winv = nullptr)
 {
     static_assert(OPK == MFMA_NATIVE || std::is_same<TC, float>::value,
                   "half-width operands accumulate in FP32");
@@ -73,6 +80,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     __shared__ short ssig[SMAX * NMAX]; // block t: logical row -> storage row
     __shared__ __align__(16) TC scand[2 * NW * FNB];
     __shared__ TC skey[2 * NW];
+    __shared__ TC spv[2 * NW];
     __shared__ int slab[2 * NW];
 
     const int tid = threadIdx.x;
@@ -171,7 +179,10 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 for(int e = 0; e < LTT; ++e)
                 {
                     const int idx = tid + e * NMAX, i = idx % FNB, k = idx / FNB;
-                    preL[e] = Aout.load(Aout.offset(true, r0 + i, r0 + k), 0); // L_tt (final)
+                    if constexpr(INVW)
+                        preL[e] = winv[t * FNB * FNB + idx]; // inv(L_tt)(i, k), column-major
+                    else
+                        preL[e] = Aout.load(Aout.offset(true, r0 + i, r0 + k), 0); // L_tt (final)
                 }
         };
         if(s > 0)
@@ -220,33 +231,81 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 }
                 if(kc == 0)
                 {
-                    // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i
-                    // of CPT right-hand columns; x_k reaches the group through
-                    // v_readlane (both halves of the wave read, each keeps its own)
-                    constexpr int CPT = FNB * FNB / NMAX;
-                    const int i = tid & (FNB - 1);
-                    const int j0 = (tid / FNB) * CPT;
-                    TC x[CPT];
-#pragma unroll
-                    for(int q = 0; q < CPT; ++q)
-                        x[q] = sU[i * UB + j0 + q];
-#pragma unroll
-                    for(int k = 0; k < FNB - 1; ++k)
+                    if constexpr(INVW)
                     {
-                        const TC lik = sLtt[i * UB + k];
-#pragma unroll
-                        for(int q = 0; q < CPT; ++q)
+                        // U_ts = inv(L_tt) C_t on the matrix cores.  Wave w < 2
+                        // forms columns [16w, 16w + 16) of U_ts: rows 0-15 from
+                        // k < 16 (inv(L_tt) is lower triangular), rows 16-31
+                        // from k < 32.  It reads only its own columns of sU
+                        // (all of them before it writes), so the result goes
+                        // back in place without a further barrier.
+                        if(w < 2)
                         {
-                            const TC x_lo = readlane_t(x[q], k);
-                            const TC x_hi = readlane_t(x[q], FNB + k);
-                            const TC xk = (lane & FNB) ? x_hi : x_lo;
-                            if(i > k)
-                                x[q] = fma_t(-lik, xk, x[q]);
+                            const int c = 16 * w + (lane & 15);
+                            const int kq = lane >> 4;
+                            const int ri = lane & 15;
+                            TC bv[FNB / 4], a0[FNB / 8], a1[FNB / 4];
+#pragma unroll
+                            for(int ks = 0; ks < FNB / 4; ++ks)
+                                bv[ks] = sU[(4 * ks + kq) * UB + c];
+#pragma unroll
+                            for(int ks = 0; ks < FNB / 8; ++ks)
+                                a0[ks] = sLtt[ri * UB + 4 * ks + kq];
+#pragma unroll
+                            for(int ks = 0; ks < FNB / 4; ++ks)
+                                a1[ks] = sLtt[(16 + ri) * UB + 4 * ks + kq];
+                            acc_t u0, u1;
+#pragma unroll
+                            for(int r = 0; r < 4; ++r)
+                            {
+                                u0[r] = TC(0);
+                                u1[r] = TC(0);
+                            }
+#pragma unroll
+                            for(int ks = 0; ks < FNB / 4; ++ks)
+                            {
+                                if(ks < FNB / 8)
+                                    u0 = M::run(a0[ks], bv[ks], u0);
+                                u1 = M::run(a1[ks], bv[ks], u1);
+                            }
+#pragma unroll
+                            for(int r = 0; r < 4; ++r)
+                            {
+                                sU[M::row(lane, r) * UB + c] = u0[r];
+                                sU[(16 + M::row(lane, r)) * UB + c] = u1[r];
+                            }
                         }
                     }
+                    else
+                    {
+                        // U_ts = L_tt^-1 C_t: lane i of each 32-lane group owns row i
+                        // of CPT right-hand columns; x_k reaches the group through
+                        // v_readlane (both halves of the wave read, each keeps its own)
+                        constexpr int CPT = FNB * FNB / NMAX;
+                        const int i = tid & (FNB - 1);
+                        const int j0 = (tid / FNB) * CPT;
+                        TC x[CPT];
 #pragma unroll
-                    for(int q = 0; q < CPT; ++q)
-                        sU[i * UB + j0 + q] = x[q];
+                        for(int q = 0; q < CPT; ++q)
+                            x[q] = sU[i * UB + j0 + q];
+#pragma unroll
+                        for(int k = 0; k < FNB - 1; ++k)
+                        {
+                            const TC lik = sLtt[i * UB + k];
+#pragma unroll
+                            for(int q = 0; q < CPT; ++q)
+                            {
+                                const TC x_lo = readlane_t(x[q], k);
+                                const TC x_hi = readlane_t(x[q], FNB + k);
+                                const TC xk = (lane & FNB) ? x_hi : x_lo;
+                                if(i > k)
+                                    x[q] = fma_t(-lik, xk, x[q]);
+                            }
+                        }
+#pragma unroll
+                        for(int q = 0; q < CPT; ++q)
+                            sU[i * UB + j0 + q] = x[q];
+                    }
                     __syncthreads();
                     if(w == wt)
                     {
@@ -353,7 +412,8 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 if(label == lab)
                 {
                     // whole 16-byte groups from the one holding column k
-                    // (aligned 16-byte stores; entries left of k are unused)
+                    // (aligned 16-byte stores; entries left of k are unused),
+                    // and the candidate's key, row and pivot value
                     constexpr int G = 16 / int(sizeof(TC));
 #pragma unroll
                     for(int j = 0; j < FNB; ++j)
@@ -361,57 +421,50 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                             scand[(par * NW + w) * FNB + j] = rowv[j];
                     skey[par * NW + w] = key;
                     slab[par * NW + w] = lab;
+                    spv[par * NW + w] = rowv[k];
                 }
                 __syncthreads();
                 // every wavefront's candidate read first, then a branch-free
-                // selection (ties to the lower row)
-                TC kws[NW];
+                // selection (ties to the lower row; bitwise operators, so no
+                // short-circuit branches)
+                TC kws[NW], pvs[NW];
                 int lws[NW];
 #pragma unroll
                 for(int ww = 0; ww < NW; ++ww)
                 {
                     kws[ww] = skey[par * NW + ww];
                     lws[ww] = slab[par * NW + ww];
+                    pvs[ww] = spv[par * NW + ww];
                 }
                 int wb = 0;
                 TC bk = kws[0];
                 int bl = lws[0];
+                TC pv = pvs[0];
 #pragma unroll
                 for(int ww = 1; ww < NW; ++ww)
                 {
-                    const bool better = kws[ww] > bk || (kws[ww] == bk && lws[ww] < bl);
+                    const bool better = (kws[ww] > bk) | ((kws[ww] == bk) & (lws[ww] < bl));
                     bk = better ? kws[ww] : bk;
                     bl = better ? lws[ww] : bl;
+                    pv = better ? pvs[ww] : pv;
                     wb = better ? ww : wb;
                 }
                 // the winner's row (its offset held in a vector register: as a
-                // scalar, every read of it waits for the previous one)
+                // scalar, every read of it waits for the previous one); the
+                // reciprocal of the pivot is formed while those reads are in
+                // flight
                 uint32_t poff = uint32_t((par * NW + wb) * FNB);
                 opaque(poff);
                 const TC* prow = scand + poff;
-                const TC pv = prow[k];
                 const bool nz = pv != TC(0);
+                const TC rp = nz ? TC(1) / pv : TC(1);
                 if(!nz && zero_at == 0)
                     zero_at = c + 1;
                 if(tid == 0)
                     ipiv[c] = bl + 1;
-                if(bl != c)
-                {
-                    // interchange logical rows c and bl in the row maps: thread
-                    // SMAX keeps spi, thread tt < s the map of block tt
-                    if(tid == SMAX)
-                    {
-                        const int tp = spi[c];
-                        spi[c] = spi[bl];
-                        spi[bl] = tp;
-                    }
-                    else if(tid < s)
-                    {
-                        const short q = ssig[tid * NMAX + c];
-                        ssig[tid * NMAX + c] = ssig[tid * NMAX + bl];
-                        ssig[tid * NMAX + bl] = q;
-                    }
-                }
+                // the interchange of logical rows c and bl moves labels only;
+                // the row maps take this block column's interchanges at once
+                // after the panel
                 if(label == bl)
                     label = c;
                 else if(label == c)
@@ -421,7 +474,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                     TC l = rowv[k];
                     if(nz)
                     {
-                        l *= TC(1) / pv;
+                        l *= rp;
                         rowv[k] = l;
                     }
                     // 8-column chunks: a chunk's reads are in flight while the
@@ -453,6 +506,77 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 Aout.store(rowv[j], off, 0); // columns past n are dropped
                 off += step;
                 opaque(off);
+            }
+        }
+        // ---- the block column's interchanges reach the row maps in one
+        // parallel step: the row at logical position tid when the panel began
+        // is now at logical position label, so every map M (spi, and ssig of
+        // each earlier block column) becomes M'[label] = M[tid] -- read
+        // before the barrier below, written after it.  With INVW the same
+        // barrier publishes the rows of L_ss, staged in LDS by columns
+        // (sbuf[m][i] = L(i, m), i > m), for the inversion that follows.
+        const bool inv_now = INVW && s + 1 < S; // uniform; the last block has no later user
+        if(inv_now)
+        {
+            const int i = label - c0;
+            if(i > 0 && i < FNB)
+#pragma unroll
+                for(int m = 0; m < FNB - 1; ++m)
+                    if(m < i)
+                        sbuf[m * FNB + i] = rowv[m];
+        }
+        {
+            const int mspi = spi[tid];
+            short msig[SMAX];
+#pragma unroll
+            for(int t = 0; t < SMAX; ++t)
+                if(t < s)
+                    msig[t] = ssig[t * NMAX + tid];
+            __syncthreads();
+            spi[label] = mspi;
+#pragma unroll
+            for(int t = 0; t < SMAX; ++t)
+                if(t < s)
+                    ssig[t * NMAX + label] = msig[t];
+        }
+        if constexpr(INVW)
+        {
+            // inv(L_ss) for the later block columns' U = inv(L) C: one
+            // wavefront solves L X = I by columns -- lane j (< 32) holds
+            // column j of X and runs the column-oriented forward substitution,
+            // x_i -= L(i, m) x_m for i > m, the L entries read as LDS
+            // broadcasts -- and writes the whole 32x32 X (ones on the
+            // diagonal, zeros above) column-major to block s of winv.
+            if(inv_now && w == s % NW) // one wavefront; the others wait at the next barrier
+            {
+                // thread id made opaque here: the identity columns would
+                // otherwise be formed once at entry and kept live (spilled)
+                const int j = fresh_tid() & (FNB - 1);
+                TC x[FNB];
+#pragma unroll
+                for(int r = 0; r < FNB; ++r)
+                    x[r] = r == j ? TC(1) : TC(0);
+                uint32_t base = 0;
+#pragma unroll
+                for(int m = 0; m < FNB - 1; ++m)
+                {
+                    // the column's reads wait for x_m (the empty asm ties
+                    // their address to it), or the unrolled solve hoists
+                    // all 496 reads to its top
+                    asm volatile("" : "+v"(base) : "v"(x[m]));
+                    const TC* col = sbuf + base + m * FNB;
+#pragma unroll
+                    for(int r = m + 1; r < FNB; ++r)
+                        x[r] = fma_t(-col[r], x[m], x[r]);
+                }
+                if(lane < FNB)
+                {
+                    // element stores: winv need not be 16-byte aligned
+                    TC* dst = winv + s * FNB * FNB + j * FNB;
+#pragma unroll
+                    for(int r = 0; r < FNB; ++r)
+                        dst[r] = x[r];
+                }
             }
         }
         ssig[s * NMAX + tid] = short(tid);
@@ -517,14 +641,14 @@ __global__ __launch_bounds__(NMAX, (std::is_same<TC, float>::value && OPK == MFM
                                                         int bc, const int* __restrict__ skip,
                                                         const double* __restrict__ rs,
                                                         const double* __restrict__ cs,
-                                                        int64_t strideS)
+                                                        int64_t strideS, Batch<TC> WI)
 {
     const int64_t b = blockIdx.x;
     if(skip && skip[b] != 0)
         return;
-    const int zero_at = lu_fused_body<TI, TC, NMAX, OPK>(
+    const int zero_at = lu_fused_body<TI, TC, NMAX, OPK, true>(
         n, AI[b], ldi, AO[b], ldo, ipivA + b * strideP, rs ? rs + b * strideS : nullptr,
-        cs ? cs + b * strideS : nullptr);
+        cs ? cs + b * strideS : nullptr, WI[b]);
     if(threadIdx.x == 0)
         infoA[b] = zero_at;
 }
@@ -532,15 +656,18 @@ __global__ __launch_bounds__(NMAX, (std::is_same<TC, float>::value && OPK == MFM
 template <typename TI, typename TC, int OPK = MFMA_NATIVE>
 rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                int ldo, int* ipiv, int64_t strideP, int* info, int bc,
-                               const int* skip, const double* rs = nullptr,
+                               Batch<TC> WI, const int* skip, const double* rs = nullptr,
                                const double* cs = nullptr, int64_t strideS = 0)
 {
     if(bc <= 0 || n <= 0)
         return rocblas_status_success;
+    if(!WI.base && !WI.ptrs)
+        return rocblas_status_internal_error; // the inverse blocks need their workspace
     auto go = [&](auto nmax) {
         constexpr int NMAX = decltype(nmax)::value;
         hipLaunchKernelGGL((lu_fused_kernel<TI, TC, NMAX, OPK>), dim3(bc), dim3(NMAX), 0, stream,
-                           n, AI, ldi, AO, ldo, ipiv, strideP, info, bc, skip, rs, cs, strideS);
+                           n, AI, ldi, AO, ldo, ipiv, strideP, info, bc, skip, rs, cs, strideS,
+                           WI);
     };
     if(n <= 128)
         go(std::integral_constant<int, 128>{});

@@ -7,9 +7,9 @@ namespace rsa
 template <>
 rocblas_status lu_fused<double, double>(hipStream_t stream, int n, Batch<double> AI, int ldi, Batch<double> AO,
                                 int ldo, int* ipiv, int64_t strideP, int* info, int bc,
-                                const int* skip)
+                                Batch<double> WI, const int* skip)
 {
     return lu_fused_launch<double, double>(stream, n, AI, ldi, AO, ldo, ipiv, strideP, info, bc,
-                                             skip);
+                                             WI, skip);
 }
 } // namespace rsa

@@ -7,9 +7,9 @@ namespace rsa
 template <>
 rocblas_status lu_fused<double, float>(hipStream_t stream, int n, Batch<double> AI, int ldi, Batch<float> AO,
                                 int ldo, int* ipiv, int64_t strideP, int* info, int bc,
-                                const int* skip)
+                                Batch<float> WI, const int* skip)
 {
     return lu_fused_launch<double, float>(stream, n, AI, ldi, AO, ldo, ipiv, strideP, info, bc,
-                                             skip);
+                                             WI, skip);
 }
 } // namespace rsa

@@ -7,15 +7,16 @@ namespace rsa
 {
 rocblas_status lu_fused_lowp(hipStream_t stream, int opk, int n, Batch<double> AI, int ldi,
                              Batch<float> AO, int ldo, int* ipiv, int64_t strideP, int* info,
-                             int bc, const double* rs, const double* cs, int64_t strideS)
+                             int bc, Batch<float> WI, const double* rs, const double* cs,
+                             int64_t strideS)
 {
     if(opk == MFMA_F16)
         return lu_fused_launch<double, float, MFMA_F16>(stream, n, AI, ldi, AO, ldo, ipiv,
-                                                        strideP, info, bc, nullptr, rs, cs,
+                                                        strideP, info, bc, WI, nullptr, rs, cs,
                                                         strideS);
     if(opk == MFMA_BF16)
         return lu_fused_launch<double, float, MFMA_BF16>(stream, n, AI, ldi, AO, ldo, ipiv,
-                                                         strideP, info, bc, nullptr, rs, cs,
+                                                         strideP, info, bc, WI, nullptr, rs, cs,
                                                          strideS);
     return rocblas_status_invalid_value;
 }

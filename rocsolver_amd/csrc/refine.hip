@@ -549,9 +549,9 @@ size_t dsgesv_workspace_bytes(int n, int nrhs, int bc, int itermax)
 {
     (void)nrhs;
     (void)itermax;
-    const size_t per
-        = sizeof(double) * (size_t(n) * n + 2 * size_t(n)) + sizeof(int) * (size_t(n) + 2);
-    return per * size_t(bc) + 6 * 256;
+    const size_t per = sizeof(double) * (size_t(n) * n + 2 * size_t(n) + size_t(lu_fused_inv_elems(n)))
+                       + sizeof(int) * (size_t(n) + 2);
+    return per * size_t(bc) + 7 * 256;
 }
 
 template <int NMAX, int NR>
@@ -616,12 +616,18 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
     int* fskip = reinterpret_cast<int*>(take(sizeof(int) * bc));
     double* rs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
     double* cs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
+    // inverted diagonal blocks of the FP64 fallback LU (the FP32 LU keeps
+    // its own in the unused second half of each slot)
+    double* winv64
+        = reinterpret_cast<double*>(take(sizeof(double) * size_t(lu_fused_inv_elems(n)) * bc));
     if(factor == 0)
         rs = cs = nullptr;
 
     const int64_t slot = int64_t(n) * n; // LU slot per matrix, in doubles
     Batch<float> LU32 = strided(reinterpret_cast<float*>(lu), 2 * slot);
+    Batch<float> WI32 = strided(reinterpret_cast<float*>(lu) + slot, 2 * slot);
     Batch<double> LU64 = strided(lu, slot);
+    Batch<double> WI64 = strided(winv64, lu_fused_inv_elems(n));
     rocblas_status st;
 #define RSA_TRY(x)                        \
     do                                    \
@@ -641,9 +647,10 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
     auto factor_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
         if(factor == 0)
             return lu_fused<double, float>(st_, n, sub(A, k0), lda, sub(LU32, k0), n,
-                                           ipiv + k0 * n, n, info32 + k0, cnt);
+                                           ipiv + k0 * n, n, info32 + k0, cnt, sub(WI32, k0));
         return lu_fused_lowp(st_, factor, n, sub(A, k0), lda, sub(LU32, k0), n, ipiv + k0 * n, n,
-                             info32 + k0, cnt, rs + k0 * n, cs + k0 * n, int64_t(n));
+                             info32 + k0, cnt, sub(WI32, k0), rs + k0 * n, cs + k0 * n,
+                             int64_t(n));
     };
     auto refine_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
         // diagonal blocks of the FP32 factors -> their inverses (used by every
@@ -735,7 +742,8 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
         RSA_TRY(refine_range(stream, 0, bc));
     }
     // FP64 fallback for the systems that did not converge
-    RSA_TRY((lu_fused<double, double>(stream, n, A, lda, LU64, n, ipiv, n, info, bc, fskip)));
+    RSA_TRY((lu_fused<double, double>(stream, n, A, lda, LU64, n, ipiv, n, info, bc, WI64,
+                                      fskip)));
     hipLaunchKernelGGL(ir_mark_singular_kernel, dim3(ceil_div(bc, 256)), dim3(256), 0, stream,
                        info, fskip, bc);
     RSA_LAUNCH_CHECK();

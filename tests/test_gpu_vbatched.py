@@ -1,11 +1,18 @@
 """GPU checks of the variable-size batched routines
 (rocsolver_amd_{s,d}{getrf,getrs,gesv}_vbatched).
 
-Every matrix runs through the kernels of the fixed-size path for its order,
-so each result must equal, bit for bit, what the fixed-size strided routine
-gives that matrix alone; the fixed-size routines are themselves pinned
-against the oracle (test_gpu_lu.py).  The oracle checks here cover the
-pivots (exact) and the solution (tolerance) directly as well.
+Every matrix runs through the device code of the fixed-size path for its
+order.  Orders up to 64 (the register kernels) and every solve with given
+factors must equal, bit for bit, what the fixed-size strided routine gives
+that matrix alone.  For larger orders the factorisation differs in one
+place: the fixed-size kernel forms its block solves U = L_tt^-1 C as
+inv(L_tt) C on the matrix cores (the inverses kept in workspace), the
+persistent kernel, which has no per-matrix workspace, by substitution.
+There the pivots and info must still be equal, the factors agree to a few
+ulps of the data, and both solutions meet the reference's backward-error
+threshold.  The fixed-size routines are themselves pinned against the
+oracle (test_gpu_lu.py); the oracle checks here cover the pivots (exact)
+and the solution (tolerance) directly as well.
 """
 import numpy as np
 import pytest
@@ -37,6 +44,40 @@ def load(torch, n, nrhs, As, Bs, dtype):
     return vb
 
 
+SMALL_MAX = 64  # register-kernel orders: bit-identical to the fixed-size path
+
+
+def assert_same_factorisation(k, LUv, LUf, ipv, ipf, a, dtype, what):
+    """Factors of one matrix a from the persistent kernel (LUv, ipv) against
+    the fixed-size routine's (LUf, ipf): bit for bit for the register-kernel
+    orders; else, where the oracle finds every pivot unambiguous (a rounding
+    difference cannot flip it), equal pivots and factors within 64 k ulps of
+    max |LU| (elsewhere the backward errors of the solutions decide)."""
+    if k <= SMALL_MAX:
+        np.testing.assert_array_equal(ipv, ipf, err_msg=what)
+        np.testing.assert_array_equal(LUv, LUf, err_msg=what)
+        return
+    eps = np.finfo(dtype).eps
+    _, _, gap = oracle.getf2(a[None].copy(), k, with_gap=True)
+    if (gap > 1e3 * k * eps).all():
+        np.testing.assert_array_equal(ipv, ipf, err_msg=what)
+        err = np.abs(LUv.astype(np.float64) - LUf).max()
+        assert err <= 64 * k * eps * max(np.abs(LUf).max(), 1.0), (what, err)
+
+
+def assert_same_solution(k, Xv, Xf, a, bm, dtype, what, trans="N"):
+    """Solutions of one system: bit for bit for the register-kernel orders,
+    else both within the reference's backward-error threshold."""
+    if k <= SMALL_MAX:
+        np.testing.assert_array_equal(Xv, Xf, err_msg=what)
+        return
+    for X in (Xv, Xf):
+        be = oracle.solve_backward_error(a[None].astype(np.float64), k,
+                                         X[None].astype(np.float64),
+                                         bm[None].astype(np.float64), trans).max()
+        assert be <= 10 * k * np.finfo(dtype).eps, (what, be)
+
+
 def fixed_gesv(torch, handle, a, bm):
     A = torch.from_numpy(a[None].copy()).cuda()
     B = torch.from_numpy(bm[None].copy()).cuda()
@@ -57,9 +98,10 @@ def test_gesv_vbatched_equals_fixed_size(torch_gpu, handle, dtype):
     info = vb.info.cpu().numpy()
     for b in range(len(n)):
         LU, X, ipiv, inf = fixed_gesv(torch, handle, As[b], Bs[b])
-        np.testing.assert_array_equal(vb.A_of(b).cpu().numpy(), LU, err_msg=f"n={n[b]}")
-        np.testing.assert_array_equal(vb.B_of(b).cpu().numpy(), X, err_msg=f"n={n[b]}")
-        np.testing.assert_array_equal(vb.ipiv_of(b).cpu().numpy(), ipiv)
+        k = int(n[b])
+        assert_same_factorisation(k, vb.A_of(b).cpu().numpy(), LU, vb.ipiv_of(b).cpu().numpy(),
+                                  ipiv, As[b], dtype, f"n={k}")
+        assert_same_solution(k, vb.B_of(b).cpu().numpy(), X, As[b], Bs[b], dtype, f"n={k}")
         assert info[b] == inf
         # and against the oracle directly
         k = int(n[b])
@@ -114,10 +156,12 @@ def test_getrf_getrs_vbatched_equal_fixed_size(torch_gpu, handle, dtype, trans):
         ipiv, _ = ra.getrf(handle, A)
         ra.getrs(handle, A, ipiv, B, trans)
         torch.cuda.synchronize()
-        np.testing.assert_array_equal(vb.A_of(b).cpu().numpy(), A[0].cpu().numpy())
-        np.testing.assert_array_equal(vb.ipiv_of(b).cpu().numpy(), ipiv[0].cpu().numpy())
-        np.testing.assert_array_equal(vb.B_of(b).cpu().numpy(), B[0].cpu().numpy(),
-                                      err_msg=f"n={n[b]}")
+        k = int(n[b])
+        assert_same_factorisation(k, vb.A_of(b).cpu().numpy(), A[0].cpu().numpy(),
+                                  vb.ipiv_of(b).cpu().numpy(), ipiv[0].cpu().numpy(), As[b],
+                                  dtype, f"n={k}")
+        assert_same_solution(k, vb.B_of(b).cpu().numpy(), B[0].cpu().numpy(), As[b], Bs[b],
+                             dtype, f"n={k} {trans}", trans)
 
 
 def test_gesv_vbatched_solution_accuracy(torch_gpu, handle):
