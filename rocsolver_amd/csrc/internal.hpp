@@ -51,10 +51,16 @@ rocblas_status lu_fused(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<
 // FP32 LU of diag(rs) A diag(cs) (A FP64; the scales of matrix b at
 // rs + b * strideS, cs + b * strideS) whose trailing updates run on the matrix
 // cores with FP16 (opk = 1) or BF16 (opk = 2) operands and FP32 accumulation.
+// Both return the factors with every 32x32 diagonal block inverted in place
+// (strictly lower part inv(L_tt), upper part with the diagonal inv(U_tt);
+// a partial last block as diag(block, I)), the form the refinement's
+// triangular solves use; the plain FP32 LU of an FP64 matrix otherwise.
 rocblas_status lu_fused_lowp(hipStream_t stream, int opk, int n, Batch<double> AI, int ldi,
                              Batch<float> AO, int ldo, int* ipiv, int64_t strideP, int* info,
-                             int bc, Batch<float> WI, const double* rs, const double* cs,
-                             int64_t strideS);
+                             int bc, const double* rs, const double* cs, int64_t strideS);
+rocblas_status lu_fused_invdiag(hipStream_t stream, int n, Batch<double> AI, int ldi,
+                                Batch<float> AO, int ldo, int* ipiv, int64_t strideP, int* info,
+                                int bc);
 
 // One-workgroup-per-system solve with the LU of getrf for SMALL_MAX < n <=
 // FUSED_MAX, arithmetic in TL; reads the right-hand sides from B (type TB),

@@ -47,11 +47,16 @@ constexpr int FHC = 16; // columns per LDS round of the staging/transposition bu
 // selects the matrix-core operands of the trailing updates: the factor
 // precision itself, or FP16 / BF16 operands with FP32 accumulation (TC =
 // float; the panel, the block solves and the stored factors stay FP32).
-// INVW: the block solves U = L_tt^-1 C run as inv(L_tt) C on the matrix
-// cores, inv(L_tt) of every block column but the last kept in winv (32x32
-// column-major blocks, block t at winv + 1024 t); otherwise each is a
+// INVM (inverse mode) 1 or 2: the block solves U = L_tt^-1 C run as
+// inv(L_tt) C on the matrix cores.  INVM 1 keeps inv(L_tt) of every block
+// column but the last in winv (32x32 column-major blocks, block t at
+// winv + 1024 t) and returns the plain factors.  INVM 2 (the internal
+// factors of the mixed-precision solvers) returns every 32x32 diagonal block
+// inverted instead -- strictly lower part inv(L_tt), upper part with the
+// diagonal inv(U_tt) -- and reads inv(L_tt) back from there.  INVM 0: each
+// block solve is a
 // substitution with the solved values passed between lanes by v_readlane.
-template <typename TI, typename TC, int NMAX, int OPK = MFMA_NATIVE, bool INVW = false>
+template <typename TI, typename TC, int NMAX, int OPK = MFMA_NATIVE, int INVM = 0>
 __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, int ldi,
                                              TC* __restrict__ Ao, int ldo, int* __restrict__ ipiv,
                                              const double* __restrict__ rs = nullptr,
@@ -60,6 +65,8 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 {
     static_assert(OPK == MFMA_NATIVE || std::is_same<TC, float>::value,
                   "half-width operands accumulate in FP32");
+    static_assert(INVM >= 0 && INVM <= 2, "inverse mode");
+    constexpr bool INVW = INVM != 0;
     using M = mfma16<TC>;
     using acc_t = typename M::acc_t;
     constexpr int NW = NMAX / 64;
@@ -179,8 +186,18 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                 for(int e = 0; e < LTT; ++e)
                 {
                     const int idx = tid + e * NMAX, i = idx % FNB, k = idx / FNB;
-                    if constexpr(INVW)
+                    if constexpr(INVM == 1)
                         preL[e] = winv[t * FNB * FNB + idx]; // inv(L_tt)(i, k), column-major
+                    else if constexpr(INVM == 2)
+                    {
+                        // strictly lower part of the inverted diagonal block;
+                        // unit diagonal, zeros above (lane positions from an
+                        // opaque thread id: kept from entry they would spill)
+                        const int x2 = fresh_tid() + e * NMAX;
+                        const int i2 = x2 % FNB, k2 = x2 / FNB;
+                        const TC v = Aout.load(Aout.offset(i2 > k2, r0 + i2, r0 + k2), 0);
+                        preL[e] = i2 == k2 ? TC(1) : v;
+                    }
                     else
                         preL[e] = Aout.load(Aout.offset(true, r0 + i, r0 + k), 0); // L_tt (final)
                 }
@@ -398,6 +415,21 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             __syncthreads();
         }
 
+        if constexpr(INVM == 2)
+        {
+            // a partial last diagonal block is inverted as diag(block, I):
+            // its two LDS matrices start as 0 (L) and I (U~), the rows of the
+            // block overwrite their part after the panel (its barriers order
+            // these writes first)
+            if(jb < FNB) // uniform
+#pragma unroll
+                for(int e = 0; e < 2 * FNB * FNB / NMAX; ++e)
+                {
+                    const int idx = fresh_tid() + e * NMAX; // opaque: not formed at entry
+                    const int rem = idx % (FNB * FNB);
+                    sbuf[idx] = (idx >= FNB * FNB && rem / FNB == rem % FNB) ? TC(1) : TC(0);
+                }
+        }
         int label = tid; // logical row held by this thread
 #pragma unroll
         for(int k = 0; k < FNB; ++k)
@@ -497,9 +529,12 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         }
 
         // ---- 4. write block column s in the logical order of this moment
+        //         (with INVM 2 the rows of the diagonal block are written by
+        //         the inversion below, so each entry is stored once)
         {
-            uint32_t off = Aout.offset(label < n, label, c0);
-            const uint32_t step = label < n ? Aout.ld_bytes : 0u;
+            const bool wr = label < n && (INVM != 2 || label < c0 || label >= c0 + jb);
+            uint32_t off = Aout.offset(wr, label, c0);
+            const uint32_t step = wr ? Aout.ld_bytes : 0u;
 #pragma unroll
             for(int j = 0; j < FNB; ++j)
             {
@@ -515,15 +550,25 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         // before the barrier below, written after it.  With INVW the same
         // barrier publishes the rows of L_ss, staged in LDS by columns
         // (sbuf[m][i] = L(i, m), i > m), for the inversion that follows.
-        const bool inv_now = INVW && s + 1 < S; // uniform; the last block has no later user
+        // With INVM 2 the upper triangle goes to a second LDS matrix in its
+        // index-reversed lower form U~ (U~(31 - k, 31 - i) = U(i, k), k >= i).
+        // uniform; with INVM 1 the last block has no later user
+        const bool inv_now = INVM == 2 || (INVM == 1 && s + 1 < S);
         if(inv_now)
         {
             const int i = label - c0;
-            if(i > 0 && i < FNB)
+            if(i >= 0 && i < jb)
+            {
 #pragma unroll
                 for(int m = 0; m < FNB - 1; ++m)
                     if(m < i)
                         sbuf[m * FNB + i] = rowv[m];
+                if constexpr(INVM == 2)
+#pragma unroll
+                    for(int k = 0; k < FNB; ++k)
+                        if(k >= i && k < jb)
+                            sbuf[FNB * FNB + (FNB - 1 - k) * FNB + (FNB - 1 - i)] = rowv[k];
+            }
         }
         {
             const int mspi = spi[tid];
@@ -551,31 +596,70 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             {
                 // thread id made opaque here: the identity columns would
                 // otherwise be formed once at entry and kept live (spilled)
-                const int j = fresh_tid() & (FNB - 1);
+                const int ft = fresh_tid();
+                const int j = ft & (FNB - 1);
+                // INVM 2: lanes 32..63 invert U~ the same way, dividing by its
+                // diagonal (x_m /= d_m; d_m = 1 for L, an exact division)
+                const int h = INVM == 2 ? (ft >> 5) & 1 : 0;
                 TC x[FNB];
 #pragma unroll
                 for(int r = 0; r < FNB; ++r)
                     x[r] = r == j ? TC(1) : TC(0);
-                uint32_t base = 0;
+                uint32_t base = uint32_t(h) * uint32_t(FNB * FNB);
 #pragma unroll
-                for(int m = 0; m < FNB - 1; ++m)
+                for(int m = 0; m < FNB; ++m)
                 {
                     // the column's reads wait for x_m (the empty asm ties
                     // their address to it), or the unrolled solve hoists
                     // all 496 reads to its top
                     asm volatile("" : "+v"(base) : "v"(x[m]));
                     const TC* col = sbuf + base + m * FNB;
+                    if constexpr(INVM == 2)
+                    {
+                        const TC dv = col[m];
+                        x[m] = x[m] / (h ? dv : TC(1));
+                    }
 #pragma unroll
                     for(int r = m + 1; r < FNB; ++r)
                         x[r] = fma_t(-col[r], x[m], x[r]);
                 }
-                if(lane < FNB)
+                if constexpr(INVM == 1)
                 {
-                    // element stores: winv need not be 16-byte aligned
-                    TC* dst = winv + s * FNB * FNB + j * FNB;
+                    if(lane < FNB)
+                    {
+                        // element stores: winv need not be 16-byte aligned
+                        TC* dst = winv + s * FNB * FNB + j * FNB;
+#pragma unroll
+                        for(int r = 0; r < FNB; ++r)
+                            dst[r] = x[r];
+                    }
+                }
+                else
+                {
+                    // results through LDS (every lane has read its matrix)
+                    // so the block is stored a column at a time: entry (i, k)
+                    // is inv(L)(i, k) below the diagonal, inv(U)(i, k) =
+                    // inv(U~)(31 - i, 31 - k) on and above it
+                    constexpr int CB = FNB + 1;
+                    wave_sync();
 #pragma unroll
                     for(int r = 0; r < FNB; ++r)
-                        dst[r] = x[r];
+                    {
+                        if(h == 0)
+                            sbuf[r * CB + j] = x[r];
+                        else
+                            sbuf[FNB * CB + (FNB - 1 - r) * CB + (FNB - 1 - j)] = x[r];
+                    }
+                    wave_sync();
+                    const int ft2 = fresh_tid(); // opaque: addresses not formed at entry
+                    const int i = ft2 & (FNB - 1);
+#pragma unroll
+                    for(int t2 = 0; t2 < FNB / 2; ++t2)
+                    {
+                        const int k = ((ft2 >> 5) & 1) + 2 * t2;
+                        const TC v = i > k ? sbuf[i * CB + k] : sbuf[FNB * CB + i * CB + k];
+                        Aout.store(v, Aout.offset(i < jb && k < jb, c0 + i, c0 + k), 0);
+                    }
                 }
             }
         }
@@ -634,7 +718,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 // compiler spreads into 300-400 registers (1 wave per SIMD).  FP64 fits 2
 // waves (170-240 VGPRs); the FP32 factorisation with 256 or 512 threads fits
 // 4 (128 VGPRs, no scratch), so four 256-thread workgroups share a CU.
-template <typename TI, typename TC, int NMAX, int OPK>
+template <typename TI, typename TC, int NMAX, int OPK, int INVM>
 __global__ __launch_bounds__(NMAX, (std::is_same<TC, float>::value && OPK == MFMA_NATIVE && NMAX >= 256) ? 4 : 2) void lu_fused_kernel(int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                                         int ldo, int* __restrict__ ipivA,
                                                         int64_t strideP, int* __restrict__ infoA,
@@ -646,26 +730,30 @@ __global__ __launch_bounds__(NMAX, (std::is_same<TC, float>::value && OPK == MFM
     const int64_t b = blockIdx.x;
     if(skip && skip[b] != 0)
         return;
-    const int zero_at = lu_fused_body<TI, TC, NMAX, OPK, true>(
+    const int zero_at = lu_fused_body<TI, TC, NMAX, OPK, INVM>(
         n, AI[b], ldi, AO[b], ldo, ipivA + b * strideP, rs ? rs + b * strideS : nullptr,
-        cs ? cs + b * strideS : nullptr, WI[b]);
+        cs ? cs + b * strideS : nullptr, INVM == 1 ? WI[b] : nullptr);
     if(threadIdx.x == 0)
         infoA[b] = zero_at;
 }
 
-template <typename TI, typename TC, int OPK = MFMA_NATIVE>
+// INVM 1: WI holds lu_fused_inv_elems(n) elements per matrix; INVM 2: WI is
+// unused (the inverted diagonal blocks are returned in AO).
+template <typename TI, typename TC, int OPK = MFMA_NATIVE, int INVM = 1>
 rocblas_status lu_fused_launch(hipStream_t stream, int n, Batch<TI> AI, int ldi, Batch<TC> AO,
                                int ldo, int* ipiv, int64_t strideP, int* info, int bc,
                                Batch<TC> WI, const int* skip, const double* rs = nullptr,
                                const double* cs = nullptr, int64_t strideS = 0)
 {
+    static_assert(INVM == 1 || INVM == 2, "fixed-size launches keep the inverses");
     if(bc <= 0 || n <= 0)
         return rocblas_status_success;
-    if(!WI.base && !WI.ptrs)
+    if(INVM == 1 && !WI.base && !WI.ptrs)
         return rocblas_status_internal_error; // the inverse blocks need their workspace
     auto go = [&](auto nmax) {
         constexpr int NMAX = decltype(nmax)::value;
-        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, NMAX, OPK>), dim3(bc), dim3(NMAX), 0, stream,
+        hipLaunchKernelGGL((lu_fused_kernel<TI, TC, NMAX, OPK, INVM>), dim3(bc), dim3(NMAX), 0,
+                           stream,
                            n, AI, ldi, AO, ldo, ipiv, strideP, info, bc, skip, rs, cs, strideS,
                            WI);
     };

@@ -445,97 +445,6 @@ __global__ __launch_bounds__(NT) void equilibrate_kernel(int n, Batch<double> AA
     }
 }
 
-// In-place inversion of the 32x32 diagonal blocks of an FP32 LU (the
-// factorisation the refinement solves with): the strictly lower part of
-// block k becomes the inverse of its unit-lower L block, the upper part with
-// the diagonal the inverse of its U block, as the triangular-solve kernels of
-// rocBLAS do with their diagonal blocks.  One wavefront per block: lane j < 32
-// forms column j of inv(L_kk), lane 32 + j column j of inv(U~), where U~ =
-// J U_kk J (J reverses the index order) is lower triangular, so both halves
-// run the same column-oriented forward substitution -- x_m /= d_m, then
-// x_i -= a_im x_m for every i > m, 31 - m independent fmas per step -- on
-// their own matrix staged in LDS (d_m = 1 for L: an exact division).  The
-// column of inv(U) is the reversed column of inv(U~).  Rows and columns past
-// n are treated as the identity and not written.  Systems with a zero FP32
-// pivot are skipped (refinement does not solve with them).
-__global__ __launch_bounds__(64, 5) void invert_diag_blocks_kernel(int n, Batch<float> LL, int ldl,
-                                                                const int* __restrict__ info32)
-{
-    // sM[h][m][i]: column m of the half's lower-triangular matrix (h = 0: L,
-    // h = 1: U~), so a step reads its column as 16-byte broadcasts; once the
-    // solve has read it, the same LDS holds the results sC[h][row][col] of
-    // inv(L) and inv(U~) (one buffer: a one-wave workgroup's LDS sets the
-    // occupancy)
-    __shared__ __align__(16) float sbuf[2 * GBS * (GBS + 1)];
-    float(*sM)[GBS][GBS] = reinterpret_cast<float(*)[GBS][GBS]>(sbuf);
-    float(*sC)[GBS][GBS + 1] = reinterpret_cast<float(*)[GBS][GBS + 1]>(sbuf);
-    const int nblk = (n + GBS - 1) / GBS;
-    const int lane = threadIdx.x;
-    const int i = lane & (GBS - 1); // row in the load/store passes, column j in the solve
-    const int h = lane >> 5;
-    {
-        // one block per workgroup (no loop over items: the per-lane LDS
-        // addresses of the unrolled solve would be kept live across it)
-        const int64_t b = blockIdx.x / nblk;
-        const int r0 = int(blockIdx.x % nblk) * GBS;
-        if(info32[b] != 0)
-            return;
-        const int nb = n - r0 < GBS ? n - r0 : GBS;
-        const MatrixBuffer<float> M(LL[b], ldl, n);
-#pragma unroll
-        for(int t = 0; t < GBS / 2; ++t)
-        {
-            const int k = (lane >> 5) + 2 * t;
-            const bool in = i < nb && k < nb;
-            const float v = M.load(M.offset(in, r0 + i, r0 + k), 0);
-            const float e = in ? v : (i == k ? 1.f : 0.f); // block entry (i, k)
-            if(i > k)
-                sM[0][k][i] = e; // L, below the diagonal
-            else
-                sM[1][GBS - 1 - k][GBS - 1 - i] = e; // U~(31-i, 31-k) = U(i, k), i <= k
-        }
-        wave_sync();
-        const int j = i;
-        float x[GBS];
-#pragma unroll
-        for(int r = 0; r < GBS; ++r)
-            x[r] = r == j ? 1.f : 0.f;
-        uint32_t base = uint32_t(h) * uint32_t(GBS * GBS); // this half's matrix in sM
-#pragma unroll
-        for(int m = 0; m < GBS; ++m)
-        {
-            // the column's reads wait for x[m] (an empty asm ties the address
-            // to it): otherwise every read of the unrolled solve is hoisted to
-            // the top and the 496 values do not fit in registers
-            asm volatile("" : "+v"(base) : "v"(x[m]));
-            // column m of this half's matrix, rows m..31 (row m: the diagonal)
-            const float* col = sbuf + base + m * GBS;
-            const float dv = col[m]; // read by both halves (select, no branch)
-            x[m] = x[m] / (h ? dv : 1.f);
-#pragma unroll
-            for(int r = m + 1; r < GBS; ++r)
-                x[r] = fmaf(-col[r], x[m], x[r]);
-        }
-        wave_sync(); // every lane has read sM: it becomes sC
-        // the whole column into the half's own region (no per-row condition)
-        float* dst = &sC[h][0][j];
-#pragma unroll
-        for(int r = 0; r < GBS; ++r)
-            dst[r * (GBS + 1)] = x[r];
-        wave_sync();
-        // entry (i, k): inv(L)[i][k] below the diagonal; inv(U)[i][k] =
-        // inv(U~)[31-i][31-k] on and above it
-#pragma unroll
-        for(int t = 0; t < GBS / 2; ++t)
-        {
-            const int k = (lane >> 5) + 2 * t;
-            const bool in = i < nb && k < nb;
-            const float v = i > k ? sC[0][i][k] : sC[1][GBS - 1 - i][GBS - 1 - k];
-            M.store(v, M.offset(in, r0 + i, r0 + k), 0);
-        }
-    }
-}
-
 // after the FP64 fallback LU: no solve for systems whose FP64 LU is singular
 __global__ void ir_mark_singular_kernel(const int* __restrict__ info, int* __restrict__ fskip,
                                         int bc)
@@ -616,8 +525,8 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
     int* fskip = reinterpret_cast<int*>(take(sizeof(int) * bc));
     double* rs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
     double* cs = reinterpret_cast<double*>(take(sizeof(double) * size_t(n) * bc));
-    // inverted diagonal blocks of the FP64 fallback LU (the FP32 LU keeps
-    // its own in the unused second half of each slot)
+    // inverted diagonal blocks of L for the FP64 fallback LU (the FP32 LU
+    // returns its diagonal blocks inverted)
     double* winv64
         = reinterpret_cast<double*>(take(sizeof(double) * size_t(lu_fused_inv_elems(n)) * bc));
     if(factor == 0)
@@ -625,7 +534,6 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
 
     const int64_t slot = int64_t(n) * n; // LU slot per matrix, in doubles
     Batch<float> LU32 = strided(reinterpret_cast<float*>(lu), 2 * slot);
-    Batch<float> WI32 = strided(reinterpret_cast<float*>(lu) + slot, 2 * slot);
     Batch<double> LU64 = strided(lu, slot);
     Batch<double> WI64 = strided(winv64, lu_fused_inv_elems(n));
     rocblas_status st;
@@ -646,19 +554,12 @@ rocblas_status dsgesv_refine(hipStream_t stream, int factor, int n, int nrhs, Ba
     };
     auto factor_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
         if(factor == 0)
-            return lu_fused<double, float>(st_, n, sub(A, k0), lda, sub(LU32, k0), n,
-                                           ipiv + k0 * n, n, info32 + k0, cnt, sub(WI32, k0));
+            return lu_fused_invdiag(st_, n, sub(A, k0), lda, sub(LU32, k0), n, ipiv + k0 * n, n,
+                                    info32 + k0, cnt);
         return lu_fused_lowp(st_, factor, n, sub(A, k0), lda, sub(LU32, k0), n, ipiv + k0 * n, n,
-                             info32 + k0, cnt, sub(WI32, k0), rs + k0 * n, cs + k0 * n,
-                             int64_t(n));
+                             info32 + k0, cnt, rs + k0 * n, cs + k0 * n, int64_t(n));
     };
     auto refine_range = [&](hipStream_t st_, int64_t k0, int cnt) -> rocblas_status {
-        // diagonal blocks of the FP32 factors -> their inverses (used by every
-        // triangular solve of the refinement)
-        const int64_t items = int64_t((n + GBS - 1) / GBS) * cnt; // < 2^31: cnt * 16 blocks
-        hipLaunchKernelGGL(invert_diag_blocks_kernel, dim3(unsigned(items)), dim3(64), 0, st_, n,
-                           sub(LU32, k0), n, info32 + k0);
-        RSA_LAUNCH_CHECK();
         const double* rs_ = rs ? rs + k0 * n : nullptr;
         const double* cs_ = cs ? cs + k0 * n : nullptr;
         double* berr_ = berr ? berr + k0 : nullptr;
