@@ -226,7 +226,12 @@ def bench_vbatched(args) -> int:
         restore()
         lt.append(event_time(torch, loop, 1))
     loop_t = sum(lt[args.warmup:]) / args.steps
+    # orders above 64 factor with substitution block solves in the persistent
+    # kernel and with inverted diagonal blocks in the fixed-size one, so the
+    # two agree to rounding there: report the largest solution difference
+    # relative to the solution's size
     loop_same = bool(torch.equal(vb.B, X_vb) and torch.equal(vb.A, LU_vb))
+    loop_rel = float((vb.B - X_vb).abs().max() / X_vb.abs().max().clamp_min(1e-300))
 
     best = graph_t if graph_t is not None else eager_t
     out = {
@@ -253,6 +258,7 @@ def bench_vbatched(args) -> int:
         "per_size_loop_ms": round(loop_t * 1e3, 4),
         "per_size_loop_launches": len(plan),
         "per_size_loop_result_identical": loop_same,
+        "per_size_loop_max_rel_solution_diff": loop_rel,
         "speedup_vs_per_size_loop": round(loop_t / best, 3),
         "tuning_table": ra.vbatched_get_tuning(),
         "autotune": tuned,

@@ -353,7 +353,9 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 #pragma unroll
                     for(int rt = 0; rt < 4; ++rt)
                     {
-                        if(64 * w + 16 * rt >= r1) // wave-uniform
+                        // wave-uniform; tiles wholly past row n hold zeros that are
+                        // never stored
+                        if(64 * w + 16 * rt >= r1 && 64 * w + 16 * rt < n)
                         {
                             const int i = 64 * w + 16 * rt + (lane & 15);
                             const int srow = ssig[t * NMAX + i];
@@ -379,7 +381,9 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 #pragma unroll
                         for(int rt = 0; rt < 4; ++rt)
                         {
-                            if(64 * w + 16 * rt >= r1) // wave-uniform
+                            // wave-uniform; tiles wholly past row n hold zeros that are
+                            // never stored
+                            if(64 * w + 16 * rt >= r1 && 64 * w + 16 * rt < n)
                             {
                                 const int i = 64 * w + 16 * rt + (lane & 15);
                                 const TC aop
