@@ -328,3 +328,57 @@ def test_config2_ipiv_and_solve_full_batch(torch_gpu, handle):
     den = an[:, None] * Bd.abs().amax(dim=2) + B0.abs().amax(dim=2)
     be = float((rn / den).max())
     assert be <= 10 * n * EPS[np.float64], be
+
+
+@pytest.mark.parametrize("n", [20, 65, 128, 200, 256, 300, 512])
+def test_dsgetrf_fp32_factors_of_fp64_input(torch_gpu, handle, n):
+    """rocsolver_amd_dsgetrf_strided_batched: the FP32 LU of an FP64 batch
+    (the plain factors; the fused path keeps its inverted diagonal blocks in
+    workspace) equals the FP32 getrf of the rounded matrix to the reference's
+    tolerance, pivots exact where the oracle finds them unambiguous."""
+    torch = torch_gpu
+    print(f"dsgetrf n={n}", flush=True)
+    g = np.random.default_rng(n + 17)
+    bc = 4
+    A64 = g.uniform(-1, 1, size=(bc, n, n))
+    A64[1] += n * np.eye(n)  # one diagonally dominant matrix: identity pivots
+    A32 = A64.astype(np.float32)
+    Ao = A32.copy()
+    ip_o, in_o, gap = oracle.getf2(Ao, n, with_gap=True)
+    Ad = dev(torch, A64)
+    LU = torch.zeros((bc, n, n), dtype=torch.float32, device="cuda")
+    ipiv = torch.zeros((bc, n), dtype=torch.int32, device="cuda")
+    info = torch.full((bc,), -7, dtype=torch.int32, device="cuda")
+    ra.check("rocsolver_amd_dsgetrf_strided_batched", handle.value, n, Ad.data_ptr(), n, n * n,
+             LU.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), bc)
+    torch.cuda.synchronize()
+    LUg, ipg = host(LU), host(ipiv)
+    np.testing.assert_array_equal(host(info), in_o)
+    unamb = (gap > 1e3 * n * EPS[np.float32]).all(axis=1)
+    assert unamb[1]
+    np.testing.assert_array_equal(ipg[unamb], ip_o[unamb])
+    np.testing.assert_array_equal(ipg[1], np.arange(1, n + 1))
+    be = oracle.lu_backward_error(A32.astype(np.float64), LUg.astype(np.float64), ipg, n, n,
+                                  EPS[np.float32])
+    assert be.max() < 10.0, be
+    for b in np.nonzero(unamb)[0]:
+        assert rel_fro(LUg[b], Ao[b]) <= 10 * n * EPS[np.float32]
+    np.testing.assert_array_equal(host(Ad), A64)  # the FP64 input is read only
+
+
+def test_fused_getrf_workspace_query(torch_gpu, handle):
+    """The fused LU (orders 65..512) asks for ceil(n/32) inverted 32x32 blocks
+    of L per matrix; the register path (n <= 64) for none."""
+    import ctypes
+    rb = ra.rocblas()
+    h = handle.value
+    for n, bc, elem, name in ((256, 10, 8, "rocsolver_dgetrf_strided_batched"),
+                              (100, 3, 4, "rocsolver_sgetrf_strided_batched"),
+                              (32, 10, 8, "rocsolver_dgetrf_strided_batched")):
+        assert rb.rocblas_start_device_memory_size_query(h) == 0
+        st = ra.call(name, h, n, n, None, n, n * n, None, n, None, bc)
+        assert st in (9, 10)
+        size = ctypes.c_size_t()
+        assert rb.rocblas_stop_device_memory_size_query(h, ctypes.byref(size)) == 0
+        want = bc * ((n + 31) // 32) * 1024 * elem if n > 64 else 0
+        assert size.value >= want and size.value - want < 4096, (n, size.value, want)
