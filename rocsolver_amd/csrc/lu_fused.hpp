@@ -77,7 +77,11 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     // prefetch registers and LDS -- ~32 KB FP32, ~58 KB FP64 -- small)
     constexpr int KC = FNB / 2;
     constexpr int NKC = FNB / KC;
-    constexpr int SBUF = (NMAX * SB > KC * NMAX) ? NMAX * SB : KC * NMAX;
+    // column stride of the gather / L21 staging [col][row]: NMAX + 1, so
+    // the 16 columns that one MFMA-layout read takes from a row fall in 16
+    // different LDS banks (a stride of NMAX puts them all in one bank)
+    constexpr int LS = NMAX + 1;
+    constexpr int SBUF = (NMAX * SB > KC * LS) ? NMAX * SB : KC * LS;
     constexpr int LTT = FNB * FNB / NMAX; // L_tt elements staged per thread
 
     __shared__ TC sbuf[SBUF]; // gather / L21 staging [col][row], transposition [row][SB]
@@ -127,7 +131,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             }
             auto put = [&](int kc, int j, TI v) {
                 const int col = c0 + KC * kc + j;
-                sbuf[j * NMAX + tid]
+                sbuf[j * LS + tid]
                     = rs ? TC(double(v) * rs[tid < n ? tid : 0] * cs[col < n ? col : 0]) : TC(v);
             };
             auto take = [&](int kc) {
@@ -142,7 +146,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                             {
                                 const int i = 64 * w + 16 * rt + M::row(lane, r);
                                 const int j = 16 * ct - KC * kc + (lane & 15);
-                                acc[rt][ct][r] = sbuf[j * NMAX + spi[i]];
+                                acc[rt][ct][r] = sbuf[j * LS + spi[i]];
                             }
                 __syncthreads();
             };
@@ -215,7 +219,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             {
 #pragma unroll
                 for(int j = 0; j < KC; ++j)
-                    sbuf[j * NMAX + tid] = pre[j];
+                    sbuf[j * LS + tid] = pre[j];
                 if(kc == 0)
                 {
 #pragma unroll
@@ -362,7 +366,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                             float av[4];
 #pragma unroll
                             for(int e = 0; e < 4; ++e)
-                                av[e] = -sbuf[(kb + e) * NMAX + srow];
+                                av[e] = -sbuf[(kb + e) * LS + srow];
 #pragma unroll
                             for(int ct = 0; ct < 2; ++ct)
                                 acc[rt][ct] = mfma16x16x16<OPK>(av, bv[ct], acc[rt][ct]);
@@ -387,7 +391,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                             {
                                 const int i = 64 * w + 16 * rt + (lane & 15);
                                 const TC aop
-                                    = -sbuf[(kk + (lane >> 4)) * NMAX + ssig[t * NMAX + i]];
+                                    = -sbuf[(kk + (lane >> 4)) * LS + ssig[t * NMAX + i]];
 #pragma unroll
                                 for(int ct = 0; ct < 2; ++ct)
                                     acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
