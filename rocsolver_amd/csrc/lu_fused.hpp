@@ -22,7 +22,7 @@
 //      order of that moment.
 // Row interchanges never touch memory.  Rows of L21_t are left where step t
 // wrote them and later interchanges are recorded in a per-block map
-// (ssig[t][logical row] = storage row), used as a gather from LDS in step 2;
+// (ssig[t][logical row] = storage row), applied when step 2 loads L21_t;
 // a last pass moves every L21_t into the final order, so the output is the
 // LAPACK layout (A = P L U, 1-based ipiv, info = first zero pivot).
 // Output may alias the input (in-place getrf) or be a separate buffer of
@@ -81,7 +81,15 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
     // the 16 columns that one MFMA-layout read takes from a row fall in 16
     // different LDS banks (a stride of NMAX puts them all in one bank)
     constexpr int LS = NMAX + 1;
-    constexpr int SBUF = (NMAX * SB > KC * LS) ? NMAX * SB : KC * LS;
+    // column stride of the L21 staging, whose rows are stored in logical
+    // order (the row maps are applied when the block is loaded): a 16-row
+    // MFMA-layout read takes 16 consecutive rows of one column, and with
+    // 4-byte entries the next lane group's column must start 16 banks on
+    // (4 columns on for the K = 16 half-width reads) so that 32 lanes meet
+    // 32 different banks
+    constexpr int LS2
+        = NMAX + (sizeof(TC) == 8 ? 1 : (OPK == MFMA_NATIVE ? 16 : 4));
+    constexpr int SBUF = (NMAX * SB > KC * LS2) ? NMAX * SB : KC * LS2;
     constexpr int LTT = FNB * FNB / NMAX; // L_tt elements staged per thread
 
     __shared__ TC sbuf[SBUF]; // gather / L21 staging [col][row], transposition [row][SB]
@@ -176,7 +184,9 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
         auto issue = [&](int t, int kc) {
             const int r0 = t * FNB, r1 = r0 + FNB;
             const bool act = tid >= r1 && tid < n;
-            uint32_t off = Aout.offset(act, tid, r0 + KC * kc);
+            // logical row tid of L21_t from its storage row (the block's map)
+            const int srow = act ? int(ssig[t * NMAX + tid]) : 0;
+            uint32_t off = Aout.offset(act, srow, r0 + KC * kc);
             const uint32_t step = act ? Aout.ld_bytes : 0u;
 #pragma unroll
             for(int j = 0; j < KC; ++j)
@@ -219,7 +229,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
             {
 #pragma unroll
                 for(int j = 0; j < KC; ++j)
-                    sbuf[j * LS + tid] = pre[j];
+                    sbuf[j * LS2 + tid] = pre[j];
                 if(kc == 0)
                 {
 #pragma unroll
@@ -362,11 +372,10 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                         if(64 * w + 16 * rt >= r1 && 64 * w + 16 * rt < n)
                         {
                             const int i = 64 * w + 16 * rt + (lane & 15);
-                            const int srow = ssig[t * NMAX + i];
                             float av[4];
 #pragma unroll
                             for(int e = 0; e < 4; ++e)
-                                av[e] = -sbuf[(kb + e) * LS + srow];
+                                av[e] = -sbuf[(kb + e) * LS2 + i];
 #pragma unroll
                             for(int ct = 0; ct < 2; ++ct)
                                 acc[rt][ct] = mfma16x16x16<OPK>(av, bv[ct], acc[rt][ct]);
@@ -390,8 +399,7 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                             if(64 * w + 16 * rt >= r1 && 64 * w + 16 * rt < n)
                             {
                                 const int i = 64 * w + 16 * rt + (lane & 15);
-                                const TC aop
-                                    = -sbuf[(kk + (lane >> 4)) * LS + ssig[t * NMAX + i]];
+                                const TC aop = -sbuf[(kk + (lane >> 4)) * LS2 + i];
 #pragma unroll
                                 for(int ct = 0; ct < 2; ++ct)
                                     acc[rt][ct] = M::run(aop, bop[ct], acc[rt][ct]);
