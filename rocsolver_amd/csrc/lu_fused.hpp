@@ -497,13 +497,15 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
                     pv = better ? pvs[ww] : pv;
                     wb = better ? ww : wb;
                 }
-                // the winner's row (its offset held in a vector register: as a
-                // scalar, every read of it waits for the previous one); the
+                // the winner's row (its index held in a vector register: as a
+                // scalar, every read of it waits for the previous one; the
+                // row offset stays a visible multiple of FNB, so the reads
+                // are 16-byte ds_read_b128 with immediate offsets); the
                 // reciprocal of the pivot is formed while those reads are in
                 // flight
-                uint32_t poff = uint32_t((par * NW + wb) * FNB);
-                opaque(poff);
-                const TC* prow = scand + poff;
+                uint32_t wbv = uint32_t(wb);
+                opaque(wbv);
+                const TC* prow = scand + (par * NW) * FNB + wbv * FNB;
                 const bool nz = pv != TC(0);
                 const TC rp = nz ? TC(1) / pv : TC(1);
                 if(!nz && zero_at == 0)
@@ -531,10 +533,27 @@ __device__ __forceinline__ int lu_fused_body(int n, const TI* __restrict__ Ai, i
 #pragma unroll
                     for(int c8 = (k + 1) / CH * CH; c8 < FNB; c8 += CH)
                     {
+                        // whole 16-byte groups (ds_read_b128; the candidate
+                        // row was stored as whole groups from the one
+                        // holding column k, and entries left of k are unused)
+                        constexpr int G = 16 / int(sizeof(TC));
+                        typedef TC g16_t __attribute__((ext_vector_type(G)));
                         TC u[CH];
 #pragma unroll
-                        for(int q = 0; q < CH; ++q)
-                            u[q] = (c8 + q > k) ? prow[c8 + q] : TC(0);
+                        for(int g = 0; g < CH / G; ++g)
+                        {
+                            if(c8 + g * G + G - 1 > k)
+                            {
+                                const g16_t v = *reinterpret_cast<const g16_t*>(prow + c8 + g * G);
+#pragma unroll
+                                for(int e = 0; e < G; ++e)
+                                    u[g * G + e] = v[e];
+                            }
+                            else
+#pragma unroll
+                                for(int e = 0; e < G; ++e)
+                                    u[g * G + e] = TC(0);
+                        }
 #pragma unroll
                         for(int q = 0; q < CH; ++q)
                             if(c8 + q > k)
