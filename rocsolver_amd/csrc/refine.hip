@@ -66,7 +66,7 @@ __device__ __forceinline__ double pow2_scale(double m)
 // Whole refinement of one system per workgroup (n <= NMAX threads, thread i
 // owns row i).  fskip[b] = 1 when the system needs no FP64 fallback.
 template <int NMAX, int NR>
-__global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
+__global__ __launch_bounds__(NMAX, NMAX <= 128 ? 3 : NMAX <= 256 ? 4 : 1) void ir_fused_kernel(
     int n, int nrhs, Batch<double> AA, int lda, Batch<float> LL, int ldl,
     const int* __restrict__ ipivA, int64_t strideP, const int* __restrict__ info32,
     Batch<double> BB, int ldb, Batch<double> XX, int ldx, int itermax, int* __restrict__ iter,
@@ -97,12 +97,16 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
     const MatrixBuffer<double> Xbuf(XX[b], ldx, nrhs);
     const bool row = tid < n;
     const MatrixBuffer<double> Bbuf(Bm, ldb, nrhs);
-    // B[tid, c] (0 for rows past n or columns past nrhs), branch-free
-    auto bload = [&](int c) {
-        return Bbuf.load((row && c < nrhs) ? Bbuf.row_offset(tid) + Bbuf.col_offset(c)
-                                           : MatrixBuffer<double>::OUT_OF_RANGE,
+    // B[t, c] (0 for rows past n or columns past nrhs), branch-free
+    auto bload_at = [&](int t, int c) {
+        return Bbuf.load((t < n && c < nrhs) ? Bbuf.row_offset(t) + Bbuf.col_offset(c)
+                                             : MatrixBuffer<double>::OUT_OF_RANGE,
                          0);
     };
+    // the same for this thread's row, the lane index re-read at each use so
+    // the loads are not hoisted out of the refinement loop and held in
+    // registers
+    auto bload = [&](int c) { return bload_at(fresh_tid(), c); };
     const bool scaled = rsA != nullptr;
     // this row's equilibration factor and this solution component's (read
     // where used: fewer live registers)
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
     {
 #pragma unroll 8
         for(int c = 0; c < nrhs; ++c)
-            big = big || beyond_fp32(bload(c));
+            big = big || beyond_fp32(bload_at(tid, c));
     }
     if(any(big))
     {
@@ -200,8 +204,10 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 d[q] = row ? sR[src][q] : 0.f;
-            solve_lower_rows<float, NR, false, true>(n, LU, d, sx, sL);
-            solve_upper_rows<float, NR, false, true>(n, LU, d, sx, sL);
+            // next block's loads issued after each update (one 32-entry array
+            // live instead of two: 4 waves/SIMD at 256 rows)
+            solve_lower_rows<float, NR, false, true, -1, true>(n, LU, d, sx, sL);
+            solve_upper_rows<float, NR, false, true, -1, true>(n, LU, d, sx, sL);
         };
 
         {
@@ -307,9 +313,32 @@ __global__ __launch_bounds__(NMAX, NMAX <= 256 ? 3 : 1) void ir_fused_kernel(
                     if(row && c0 + q < nrhs && acc[q] != 0.0 && isfinite(acc[q]))
                         remax = fmax(remax, double(ilogb(acc[q]) + ki));
                 }
-                const double cm = wave_colmax<K>(v);
-                if(lane % LPC == 0)
-                    sred[w][lane / LPC] = cm;
+                if constexpr(NR >= 2)
+                {
+                    // |R| and |X| reduced one after the other: half the
+                    // registers of one 2 NR-column reduction
+                    constexpr int L1 = 64 / NR;
+                    double vr[NR], vx[NR];
+#pragma unroll
+                    for(int q = 0; q < NR; ++q)
+                        vr[q] = v[q];
+                    const double cr = wave_colmax<NR>(vr);
+#pragma unroll
+                    for(int q = 0; q < NR; ++q)
+                        vx[q] = fabs(sX[tid][q]);
+                    const double cx = wave_colmax<NR>(vx);
+                    if(lane % L1 == 0)
+                    {
+                        sred[w][lane / L1] = cr;
+                        sred[w][NR + lane / L1] = cx;
+                    }
+                }
+                else
+                {
+                    const double cm = wave_colmax<K>(v);
+                    if(lane % LPC == 0)
+                        sred[w][lane / LPC] = cm;
+                }
             }
             remax = group_max<64>(remax);
             if(measure)
