@@ -397,15 +397,18 @@ rocblas_status rocsolver_dlaswp(rocblas_handle handle, const rocblas_int n, doub
  * singular) are solved again with an FP64 LU.
  *   A, B   device, read only (A is not overwritten, unlike DSGESV's fallback)
  *   X      device, ldx >= n: the solution
- *   iter   device, per matrix: >= 0 corrections used; -2 FP32 overflow;
- *          -3 FP32 LU singular; -31 no convergence (FP64 path used)
+ *   iter   device, per matrix: >= 0 corrections used; -1 FP64 path by
+ *          size (below); -2 FP32 overflow; -3 FP32 LU singular; -31 no
+ *          convergence (FP64 path used)
  *   info   device, per matrix: first zero pivot of the FP64 LU on the
  *          fallback path, else 0 (X is defined only where info == 0)
  *   berr   device or NULL: per matrix max_c ||R_c||_inf / (||A||_inf ||X_c||_inf)
  *          at convergence (-1 where the FP64 path produced X)
  * Workspace comes from the handle (size query supported): about
- * bc * (8 n^2 + 4 n) bytes.  Supported: n <= 512 and
- * lda * n * 8 < 2^31 (rocblas_status_not_implemented otherwise).  With
+ * bc * (8 n^2 + 4 n) bytes.  Refinement runs for n <= 512 with A, B and X
+ * each under 2^31 bytes per matrix; otherwise every system is solved by an
+ * FP64 LU of a copy of A and iter = -1 (DSGESV: fell back to full precision
+ * for implementation-specific reasons), berr = -1.  With
  * nrhs > 8 (> 4 when n > 256) the columns are refined in groups of 8 (4):
  * each group iterates until its own columns pass the test (its solution is
  * the one a call with those columns alone gives), and iter is the largest

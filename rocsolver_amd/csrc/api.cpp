@@ -17,6 +17,7 @@
 
 #include <rocblas/internal/rocblas_device_malloc.hpp>
 
+#include <algorithm>
 #include <cstring>
 
 using namespace rsa;
@@ -302,6 +303,77 @@ rocblas_status potrs_impl(rocblas_handle handle, rocblas_fill uplo, int n, int n
 
 constexpr int DSGESV_ITERMAX = 30; // LAPACK dsgesv ITERMAX
 
+// D[b][i, c] = S[b][i, c] for i < m, c < ncols; batch entries boff + blockIdx.y
+__global__ void copy_block_kernel(int m, int ncols, Batch<double> S, int lds, Batch<double> D,
+                                  int ldd, int64_t boff)
+{
+    const int64_t b = blockIdx.y + boff;
+    const double* s = S[b];
+    double* d = D[b];
+    for(int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < int64_t(m) * ncols;
+        e += int64_t(gridDim.x) * blockDim.x)
+    {
+        const int i = int(e % m), c = int(e / m);
+        d[i + int64_t(c) * ldd] = s[i + int64_t(c) * lds];
+    }
+}
+
+__global__ void mark_fp64_only_kernel(int* iter, double* berr, int bc)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if(b < bc)
+    {
+        iter[b] = -1;
+        if(berr)
+            berr[b] = -1.0;
+    }
+}
+
+static rocblas_status copy_block(hipStream_t stream, int m, int ncols, Batch<double> S, int lds,
+                                 Batch<double> D, int ldd, int bc)
+{
+    const int gx = int(std::min<int64_t>((int64_t(m) * ncols + 255) / 256, 1024));
+    for(int64_t b0 = 0; b0 < bc; b0 += 65535)
+    {
+        const int nb = int(std::min<int64_t>(bc - b0, 65535));
+        hipLaunchKernelGGL(copy_block_kernel, dim3(gx, nb), dim3(256), 0, stream, m, ncols, S,
+                           lds, D, ldd, b0);
+        RSA_LAUNCH_CHECK();
+    }
+    return rocblas_status_success;
+}
+
+// Orders beyond the fused kernels (or a matrix beyond their 32-bit buffer
+// offsets): every system is solved by the FP64 LU of the blocked path on a
+// copy of A in the workspace, and iter = -1, LAPACK DSGESV's code for "fell
+// back to full precision for implementation- or machine-specific reasons";
+// berr = -1 as on every FP64 path.  X = B where info != 0.
+static rocblas_status dsgesv_fp64_only(hipStream_t stream, int n, int nrhs, Batch<double> A,
+                                       int lda, Batch<double> B, int ldb, Batch<double> X,
+                                       int ldx, int* iter, int* info, double* berr, int bc,
+                                       void* work)
+{
+    const int64_t slot = int64_t(n) * n;
+    double* lu = static_cast<double*>(work);
+    int* ipiv = reinterpret_cast<int*>(
+        static_cast<char*>(work) + (sizeof(double) * slot * bc + 255) / 256 * 256);
+    const Batch<double> LU = strided(lu, slot);
+    rocblas_status st = copy_block(stream, n, n, A, lda, LU, n, bc);
+    if(st == rocblas_status_success)
+        st = copy_block(stream, n, nrhs, B, ldb, X, ldx, bc);
+    if(st == rocblas_status_success)
+        st = getrf_blocked<double>(stream, n, n, LU, n, ipiv, n, info, bc, true);
+    if(st == rocblas_status_success)
+        st = getrs_blocked<double>(stream, false, n, nrhs, LU, n, ipiv, n, X, ldx, bc, true,
+                                   info);
+    if(st != rocblas_status_success)
+        return st;
+    hipLaunchKernelGGL(mark_fp64_only_kernel, dim3((bc + 255) / 256), dim3(256), 0, stream, iter,
+                       berr, bc);
+    RSA_LAUNCH_CHECK();
+    return rocblas_status_success;
+}
+
 rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, const double* A,
                            int lda, int64_t strideA, const double* B, int ldb, int64_t strideB,
                            double* X, int ldx, int64_t strideX, int* iter, int* info,
@@ -330,12 +402,15 @@ rocblas_status dsgesv_impl(rocblas_handle handle, int factor, int n, int nrhs, c
         rocblas_status st = reset_info(stream, info, bc);
         return st != rocblas_status_success ? st : reset_info(stream, iter, bc);
     }
-    if(n > FUSED_MAX || !fits_buffer<double>(lda, n)
-       || !fits_buffer<double>(ldb, nrhs) || !fits_buffer<double>(ldx, nrhs))
-        return rocblas_status_not_implemented;
     rocblas_device_malloc mem(handle, ws);
     if(!mem)
         return rocblas_status_memory_error;
+    if(n > FUSED_MAX || !fits_buffer<double>(lda, n)
+       || !fits_buffer<double>(ldb, nrhs) || !fits_buffer<double>(ldx, nrhs))
+        return dsgesv_fp64_only(stream, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
+                                strided(const_cast<double*>(B), strideB), ldb,
+                                strided(X, strideX), ldx, iter, info, berr, bc,
+                                static_cast<void*>(mem));
     return dsgesv_refine(stream, factor, n, nrhs, strided(const_cast<double*>(A), strideA), lda,
                          strided(const_cast<double*>(B), strideB), ldb, strided(X, strideX), ldx,
                          iter, info, berr, bc, DSGESV_ITERMAX, static_cast<void*>(mem));

@@ -47,6 +47,38 @@ def test_dsgesv_against_oracle(torch_gpu, handle, n, nrhs, bc):
             assert 0 <= berr[b] <= np.sqrt(n) * EPS64 * 1.0001
 
 
+@pytest.mark.parametrize("factor", ["fp32", "fp16"])
+@pytest.mark.parametrize("n,nrhs,bc", [(513, 2, 3), (640, 3, 2)])
+def test_dsgesv_beyond_fused_orders_uses_fp64(torch_gpu, handle, factor, n, nrhs, bc):
+    """n > 512: every system is solved by the FP64 LU, iter = -1 (LAPACK
+    DSGESV's code for a fall-back to full precision for implementation-specific
+    reasons), berr = -1, info as the FP64 LU's (the oracle's fall-back LU
+    finds the same first zero pivot in the singular entry)."""
+    print(f"dsgesv beyond fused n={n} nrhs={nrhs} bc={bc} factor={factor}", flush=True)
+    g = np.random.default_rng(n)
+    A = g.standard_normal((bc, n, n))
+    A[1][20, :] = 0.0  # column 20 zero -> first zero pivot at 21
+    B = g.standard_normal((bc, nrhs, n))
+    Xo, it_o, info_o = oracle.dsgesv(A, n, B)
+    Ad, Bd = dev(torch_gpu, A), dev(torch_gpu, B)
+    X, it, info, berr = ra.dsgesv(handle, Ad, Bd, factor=factor)
+    torch_gpu.cuda.synchronize()
+    X, it, info, berr = X.cpu().numpy(), it.cpu().numpy(), info.cpu().numpy(), berr.cpu().numpy()
+    np.testing.assert_array_equal(Ad.cpu().numpy(), A)
+    np.testing.assert_array_equal(Bd.cpu().numpy(), B)
+    np.testing.assert_array_equal(it, -1)
+    np.testing.assert_array_equal(berr, -1.0)
+    np.testing.assert_array_equal(info, info_o)
+    assert info[1] == 21 and info[0] == 0
+    ok = info == 0
+    be = oracle.solve_backward_error(A[ok], n, X[ok], B[ok])
+    assert be.max() <= 10 * n * EPS64, be
+    for b in np.flatnonzero(ok):
+        kappa = np.linalg.cond(A[b])
+        fe = np.abs(X[b] - Xo[b]).max() / np.abs(Xo[b]).max()
+        assert fe <= 10 * kappa * n * EPS64, (b, fe, kappa)
+
+
 def test_dsgesv_fallback_paths(torch_gpu, handle):
     g = np.random.default_rng(7)
     n, nrhs, bc = 80, 2, 4
