@@ -215,13 +215,13 @@ __global__ __launch_bounds__(NMAX) void potrs_fused_kernel(int n, int nrhs, Batc
         }
         if(up)
         {
-            solve_lower_rows<T, NR, true, false, -1, true>(n, A, x, sx, sL); // U^T y = b
-            solve_upper_rows<T, NR, false, false, -1, true>(n, A, x, sx, sL); // U x = y
+            solve_lower_rows<T, NR, true>(n, A, x, sx, sL); // U^T y = b
+            solve_upper_rows<T, NR, false>(n, A, x, sx, sL); // U x = y
         }
         else
         {
-            solve_lower_rows<T, NR, false, false, 1, true>(n, A, x, sx, sL); // L y = b
-            solve_upper_rows<T, NR, true, false, 1, true>(n, A, x, sx, sL); // L^T x = y
+            solve_lower_rows<T, NR, false, false, 1>(n, A, x, sx, sL); // L y = b
+            solve_upper_rows<T, NR, true, false, 1>(n, A, x, sx, sL); // L^T x = y
         }
         if(tid < n)
         {

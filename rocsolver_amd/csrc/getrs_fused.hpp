@@ -53,8 +53,8 @@ __device__ __forceinline__ void getrs_fused_body(int n, int nrhs, const TL* __re
                                      : MatrixBuffer<TB>::OUT_OF_RANGE;
             x[q] = TL(Bbuf.load(off, 0));
         }
-        solve_lower_rows<TL, NR, TRANS, false, -1, true>(n, A, x, sx, sL);
-        solve_upper_rows<TL, NR, TRANS, false, -1, true>(n, A, x, sx, sL);
+        solve_lower_rows<TL, NR, TRANS>(n, A, x, sx, sL);
+        solve_upper_rows<TL, NR, TRANS>(n, A, x, sx, sL);
         // every thread has read its right-hand side before any row is stored
         // (X may alias B and op = T stores at the permuted row)
         __syncthreads();

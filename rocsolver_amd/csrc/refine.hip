@@ -204,10 +204,8 @@ __global__ __launch_bounds__(NMAX, NMAX <= 128 ? 3 : NMAX <= 256 ? 4 : 1) void i
 #pragma unroll
             for(int q = 0; q < NR; ++q)
                 d[q] = row ? sR[src][q] : 0.f;
-            // next block's loads issued after each update (one 32-entry array
-            // live instead of two: 4 waves/SIMD at 256 rows)
-            solve_lower_rows<float, NR, false, true, -1, true>(n, LU, d, sx, sL);
-            solve_upper_rows<float, NR, false, true, -1, true>(n, LU, d, sx, sL);
+            solve_lower_rows<float, NR, false, true>(n, LU, d, sx, sL);
+            solve_upper_rows<float, NR, false, true>(n, LU, d, sx, sL);
         };
 
         {

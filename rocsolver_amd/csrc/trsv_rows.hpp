@@ -4,10 +4,10 @@
 //
 // Each triangle is walked in 32-row diagonal blocks (two barriers per block):
 //   * every thread whose row meets the block column loads its 32 entries of
-//     that block column into registers one block ahead of use, so the loads
-//     of block k+1 are in flight while block k is solved (LATE: issued after
-//     block k's update instead, in flight across block k+1's two barriers
-//     and diagonal solve; one register array of 32 entries instead of two);
+//     that block column into registers right after block k-1's update, so
+//     the loads are in flight across block k's two barriers and diagonal
+//     solve (one register array of 32 entries; loading a whole block ahead
+//     needs two, which halves the resident waves of the FP64 solves);
 //   * the half wave owning the diagonal block publishes its right-hand sides
 //     and its rows of the block to LDS; every half wave of the workgroup then
 //     solves one right-hand side of the block (a 32-step chain in which the
@@ -145,7 +145,7 @@ __device__ __forceinline__ void apply_inv_block(TL (*sx)[NR], const TL (*sL)[GBS
 // op N (L), non-unit for op T (U^T); DIAG = 1 forces a non-unit diagonal
 // (Cholesky factors).  sx: LDS [2][GBS][NR]; sL: LDS [GBS][GBS+1].  INV
 // (op N only): the diagonal blocks hold their inverses.
-template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1, bool LATE = false>
+template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1>
 __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
@@ -153,14 +153,12 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
     const int tid = fresh_tid();
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
-    TL cur[GBS], nxt[GBS];
+    TL cur[GBS];
     load_segment<TL, TRANS>(A, tid, 0, n, tid < n, cur);
     for(int blk = 0; blk < nblk; ++blk)
     {
         const int r0 = blk * GBS;
         const int par = blk & 1;
-        if(!LATE && blk + 1 < nblk) // rows at or below the next block meet its block column
-            load_segment<TL, TRANS>(A, tid, r0 + GBS, n, tid >= r0 + GBS && tid < n, nxt);
         const bool owner = (tid >> 5) == blk;
         if(owner)
         {
@@ -192,24 +190,15 @@ __device__ __forceinline__ void solve_lower_rows(int n, const MatrixBuffer<TL>& 
                     for(int q = 0; q < NR; ++q)
                         x[q] = fma_t(-cur[k], sx[par][k][q], x[q]);
         }
-        if constexpr(LATE)
-        {
-            if(blk + 1 < nblk)
-                load_segment<TL, TRANS>(A, tid, r0 + GBS, n, tid >= r0 + GBS && tid < n, cur);
-        }
-        else
-        {
-#pragma unroll
-            for(int k = 0; k < GBS; ++k)
-                cur[k] = nxt[k];
-        }
+        if(blk + 1 < nblk) // rows at or below the next block meet its block column
+            load_segment<TL, TRANS>(A, tid, r0 + GBS, n, tid >= r0 + GBS && tid < n, cur);
     }
 }
 
 // Backward substitution with the upper triangle of op(A): non-unit for op N
 // (U), unit diagonal for op T (L^T); DIAG = 1 forces a non-unit diagonal.
 // INV as for solve_lower_rows.
-template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1, bool LATE = false>
+template <typename TL, int NR, bool TRANS, bool INV = false, int DIAG = -1>
 __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& A, TL (&x)[NR],
                                                  TL (*sx)[GBS][NR], TL (*sL)[GBS + 1])
 {
@@ -217,7 +206,7 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
     const int tid = fresh_tid();
     const int lane32 = tid & (GBS - 1);
     const int nblk = (n + GBS - 1) / GBS;
-    TL cur[GBS], nxt[GBS];
+    TL cur[GBS];
     {
         const int r0 = (nblk - 1) * GBS;
         load_segment<TL, TRANS>(A, tid, r0, n, tid < n, cur);
@@ -226,8 +215,6 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
     {
         const int r0 = blk * GBS;
         const int par = blk & 1;
-        if(!LATE && blk > 0) // rows above this block meet the previous block column
-            load_segment<TL, TRANS>(A, tid, r0 - GBS, n, tid < r0, nxt);
         const bool owner = (tid >> 5) == blk;
         if(owner)
         {
@@ -259,17 +246,8 @@ __device__ __forceinline__ void solve_upper_rows(int n, const MatrixBuffer<TL>& 
                     for(int q = 0; q < NR; ++q)
                         x[q] = fma_t(-cur[k], sx[par][k][q], x[q]);
         }
-        if constexpr(LATE)
-        {
-            if(blk > 0)
-                load_segment<TL, TRANS>(A, tid, r0 - GBS, n, tid < r0, cur);
-        }
-        else
-        {
-#pragma unroll
-            for(int k = 0; k < GBS; ++k)
-                cur[k] = nxt[k];
-        }
+        if(blk > 0) // rows above this block meet the previous block column
+            load_segment<TL, TRANS>(A, tid, r0 - GBS, n, tid < r0, cur);
     }
 }
 
