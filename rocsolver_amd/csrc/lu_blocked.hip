@@ -4,9 +4,11 @@
 // getrf, per block column j of width jb (reference structure:
 // library/src/lapack/roclapack_getrf.hpp:608-752, without rocBLAS):
 //   1. panel_kernel: one workgroup per matrix factors A[j:m, j:j+jb] with
-//      partial pivoting (workgroup arg-max per column), swaps the pivot rows
-//      across all n columns (roclapack_getf2.hpp:41-99 semantics), then solves
-//      U12 = L11^-1 A12 for the columns right of the panel.
+//      partial pivoting (workgroup arg-max per column), interchanging rows
+//      inside the panel at each column; then applies the panel's interchanges
+//      to every other column at once (roclapack_getf2.hpp:41-99 semantics:
+//      the same final rows), and solves U12 = L11^-1 A12 for the columns
+//      right of the panel.
 //   2. gemm_sub_kernel: A22 -= L21 * U12 on MFMA (v_mfma_f64_16x16x4f64 /
 //      v_mfma_f32_16x16x4f32), 64x64 tiles.
 // getrs (roclapack_getrs.hpp:117-150): laswp, then for each 64-row diagonal
@@ -55,6 +57,10 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
     __shared__ T s_pv;
     __shared__ int s_p;
     __shared__ T s_L[PANEL_NB * PANEL_NB];
+    __shared__ int s_swap[PANEL_NB]; // row interchanged with row j0 + k (itself: none)
+    __shared__ int s_pos[2 * PANEL_NB]; // rows touched by the panel's interchanges
+    __shared__ int s_src[2 * PANEL_NB]; // original row whose entries end at s_pos[q]
+    __shared__ int s_cnt;
 
     const int64_t b = blockIdx.x;
     T* A = AA[b];
@@ -114,10 +120,14 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         const int p = s_p;
         const T pv = s_pv;
         const bool nz = pv != T(0);
-        // interchange rows c and p across all n columns
+        if(tid == 0)
+            s_swap[k] = (nz && p != c) ? p : c;
+        // interchange rows c and p inside the panel (the other columns take
+        // all of the panel's interchanges at once after it)
+        const int jend = j0 + jb;
         if(nz && p != c)
         {
-            for(int j = tid; j < n; j += PANEL_THREADS)
+            for(int j = j0 + tid; j < jend; j += PANEL_THREADS)
             {
                 T t = at(c, j);
                 at(c, j) = at(p, j);
@@ -129,18 +139,105 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         // leaves the column alone with pivoting (every candidate is zero) and
         // multiplies it by zero without (the reference npvt kernels keep the
         // scale at 0: roclapack_getf2_specialized_kernels.hpp:185-188,427-430)
-        const int jend = j0 + jb;
         const T rcp = nz ? T(1) / pv : T(0);
+        // the pivot row's panel entries right of c, then per row all loads
+        // before any store, so they are in flight together
+        T u[PANEL_NB];
+#pragma unroll
+        for(int q = 0; q < PANEL_NB; ++q)
+        {
+            const int j = j0 + q;
+            u[q] = (j > c && j < jend) ? at(c, j) : T(0);
+        }
         for(int i = c + 1 + tid; i < m; i += PANEL_THREADS)
         {
             T l = at(i, c);
+            T v[PANEL_NB];
+#pragma unroll
+            for(int q = 0; q < PANEL_NB; ++q)
+            {
+                const int j = j0 + q;
+                v[q] = (j > c && j < jend) ? at(i, j) : T(0);
+            }
             if(nz || !PIVOT)
             {
                 l *= rcp;
                 at(i, c) = l;
             }
-            for(int j = c + 1; j < jend; ++j)
-                at(i, j) = fma_t(-l, at(c, j), at(i, j));
+#pragma unroll
+            for(int q = 0; q < PANEL_NB; ++q)
+            {
+                const int j = j0 + q;
+                if(j > c && j < jend)
+                    at(i, j) = fma_t(-l, u[q], v[q]);
+            }
+        }
+        __syncthreads();
+    }
+
+    // the panel's interchanges on every column outside it: composed into one
+    // map of at most 2 jb rows (s_src[q]: the original row whose entries end
+    // in row s_pos[q]), then each column reads those rows and writes them
+    // back permuted
+    if(PIVOT && n > jb)
+    {
+        if(wid == 0)
+        {
+            // lane q keeps slot q (row pos, source src); ballots find a row's
+            // slot, a new row takes the next free one
+            int pos = -1, src = -1, cnt = 0;
+            for(int k = 0; k < jb; ++k)
+            {
+                const int r = j0 + k, r2 = s_swap[k];
+                if(r2 == r) // uniform
+                    continue;
+                int qs[2];
+#pragma unroll
+                for(int h = 0; h < 2; ++h)
+                {
+                    const int row = h ? r2 : r;
+                    const uint64_t hit = __builtin_amdgcn_ballot_w64(pos == row);
+                    int q = hit ? __builtin_ffsll(hit) - 1 : cnt;
+                    if(!hit)
+                    {
+                        if(lane == q)
+                        {
+                            pos = row;
+                            src = row;
+                        }
+                        ++cnt;
+                    }
+                    qs[h] = q;
+                }
+                const int sa = __builtin_amdgcn_readlane(src, qs[0]);
+                const int sb = __builtin_amdgcn_readlane(src, qs[1]);
+                if(lane == qs[0])
+                    src = sb;
+                if(lane == qs[1])
+                    src = sa;
+            }
+            if(lane < cnt)
+            {
+                s_pos[lane] = pos;
+                s_src[lane] = src;
+            }
+            if(lane == 0)
+                s_cnt = cnt;
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        const int nout = n - jb; // columns left and right of the panel
+        for(int e = tid; e < nout; e += PANEL_THREADS)
+        {
+            const int j = e < j0 ? e : e + jb;
+            T v[2 * PANEL_NB];
+#pragma unroll
+            for(int q = 0; q < 2 * PANEL_NB; ++q)
+                v[q] = q < cnt ? at(s_src[q], j) : T(0);
+#pragma unroll
+            for(int q = 0; q < 2 * PANEL_NB; ++q)
+                if(q < cnt)
+                    at(s_pos[q], j) = v[q];
         }
         __syncthreads();
     }
@@ -161,12 +258,23 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
 #pragma unroll
             for(int i = 0; i < PANEL_NB; ++i)
                 x[i] = i < jb ? at(j0 + i, j) : T(0);
+            // L11 is read from LDS in every column, through an offset made
+            // opaque to the optimiser every two columns of L: otherwise its
+            // 496 entries are loaded ahead all at once (or hoisted out of the
+            // column loop) and spilled
+            const T* L = s_L;
 #pragma unroll
             for(int k = 0; k < PANEL_NB; ++k)
             {
+                if((k & 1) == 0)
+                {
+                    uint32_t lo = 0;
+                    opaque(lo);
+                    L = s_L + lo;
+                }
 #pragma unroll
                 for(int i = k + 1; i < PANEL_NB; ++i)
-                    x[i] = fma_t(-s_L[i + k * PANEL_NB], x[k], x[i]);
+                    x[i] = fma_t(-L[i + k * PANEL_NB], x[k], x[i]);
             }
 #pragma unroll
             for(int i = 0; i < PANEL_NB; ++i)
