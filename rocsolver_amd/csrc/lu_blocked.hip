@@ -7,8 +7,8 @@
 //      partial pivoting (workgroup arg-max per column), interchanging rows
 //      inside the panel at each column; then applies the panel's interchanges
 //      to every other column at once (roclapack_getf2.hpp:41-99 semantics:
-//      the same final rows), and solves U12 = L11^-1 A12 for the columns
-//      right of the panel.
+//      the same final rows); u12_kernel then solves U12 = L11^-1 A12 for the
+//      columns right of the panel, one thread per column.
 //   2. gemm_sub_kernel: A22 -= L21 * U12 on MFMA (v_mfma_f64_16x16x4f64 /
 //      v_mfma_f32_16x16x4f32), 64x64 tiles.
 // getrs (roclapack_getrs.hpp:117-150): laswp, then for each 64-row diagonal
@@ -56,7 +56,6 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
     __shared__ int s_idx[NW];
     __shared__ T s_pv;
     __shared__ int s_p;
-    __shared__ T s_L[PANEL_NB * PANEL_NB];
     __shared__ int s_swap[PANEL_NB]; // row interchanged with row j0 + k (itself: none)
     __shared__ int s_pos[2 * PANEL_NB]; // rows touched by the panel's interchanges
     __shared__ int s_src[2 * PANEL_NB]; // original row whose entries end at s_pos[q]
@@ -241,47 +240,54 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         }
         __syncthreads();
     }
+}
 
-    // U12 = L11^-1 A12 for the columns right of the panel
-    const int c0 = j0 + jb;
-    if(c0 < n)
+// U12 = L11^-1 A12 for the columns right of a finished panel (L11 = the unit
+// lower triangle of A[j0:j0+jb, j0:j0+jb]): one thread per column, 64
+// columns per workgroup, L11 staged in LDS (zero outside jb x jb).
+template <typename T>
+__global__ __launch_bounds__(64) void u12_kernel(int n, Batch<T> AA, int lda, int j0, int jb,
+                                                 int64_t boff)
+{
+    __shared__ T sL[PANEL_NB * PANEL_NB];
+    const int64_t b = blockIdx.y + boff;
+    T* A = AA[b];
+    const int tid = threadIdx.x;
+    for(int e = tid; e < PANEL_NB * PANEL_NB; e += 64)
     {
-        for(int e = tid; e < jb * jb; e += PANEL_THREADS)
-        {
-            const int i = e % jb, k = e / jb;
-            s_L[i + k * PANEL_NB] = at(j0 + i, j0 + k);
-        }
-        __syncthreads();
-        for(int j = c0 + tid; j < n; j += PANEL_THREADS)
-        {
-            T x[PANEL_NB];
-#pragma unroll
-            for(int i = 0; i < PANEL_NB; ++i)
-                x[i] = i < jb ? at(j0 + i, j) : T(0);
-            // L11 is read from LDS in every column, through an offset made
-            // opaque to the optimiser every two columns of L: otherwise its
-            // 496 entries are loaded ahead all at once (or hoisted out of the
-            // column loop) and spilled
-            const T* L = s_L;
-#pragma unroll
-            for(int k = 0; k < PANEL_NB; ++k)
-            {
-                if((k & 1) == 0)
-                {
-                    uint32_t lo = 0;
-                    opaque(lo);
-                    L = s_L + lo;
-                }
-#pragma unroll
-                for(int i = k + 1; i < PANEL_NB; ++i)
-                    x[i] = fma_t(-L[i + k * PANEL_NB], x[k], x[i]);
-            }
-#pragma unroll
-            for(int i = 0; i < PANEL_NB; ++i)
-                if(i < jb)
-                    at(j0 + i, j) = x[i];
-        }
+        const int i = e % PANEL_NB, k = e / PANEL_NB;
+        sL[e] = (i < jb && k < jb) ? A[j0 + i + int64_t(j0 + k) * lda] : T(0);
     }
+    __syncthreads();
+    const int j = j0 + jb + blockIdx.x * 64 + tid;
+    if(j >= n)
+        return;
+    T* col = A + int64_t(j) * lda + j0;
+    T x[PANEL_NB];
+#pragma unroll
+    for(int i = 0; i < PANEL_NB; ++i)
+        x[i] = i < jb ? col[i] : T(0);
+    // L11 read through an offset that the optimiser must take as computed
+    // from x[k] every two columns (an empty asm tied to x[k]), so its 496
+    // entries are not all loaded ahead of the substitution (and spilled)
+    const T* L = sL;
+#pragma unroll
+    for(int k = 0; k < PANEL_NB; ++k)
+    {
+        if((k & 1) == 0)
+        {
+            uint32_t lo = 0;
+            asm volatile("" : "+v"(lo) : "v"(x[k]));
+            L = sL + lo;
+        }
+#pragma unroll
+        for(int i = k + 1; i < PANEL_NB; ++i)
+            x[i] = fma_t(-L[i + k * PANEL_NB], x[k], x[i]);
+    }
+#pragma unroll
+    for(int i = 0; i < PANEL_NB; ++i)
+        if(i < jb)
+            col[i] = x[i];
 }
 
 // ------------------------------------------------------------------ GEMM
@@ -429,6 +435,14 @@ rocblas_status getrf_blocked(hipStream_t stream, int m, int n, Batch<T> A, int l
                                m, n, A, lda, ipiv, strideP, info, j, jb);
         RSA_LAUNCH_CHECK();
         const int r0 = j + jb;
+        if(r0 < n)
+            for(int64_t b0 = 0; b0 < bc; b0 += MAX_GRID_YZ)
+            {
+                const int nb = int((bc - b0) < MAX_GRID_YZ ? (bc - b0) : MAX_GRID_YZ);
+                hipLaunchKernelGGL((u12_kernel<T>), dim3(ceil_div(n - r0, 64), nb), dim3(64), 0,
+                                   stream, n, A, lda, j, jb, b0);
+                RSA_LAUNCH_CHECK();
+            }
         if(r0 < m && r0 < n)
         {
             st = gemm_sub<T>(stream, false, m - r0, n - r0, jb, A, r0 + int64_t(j) * lda, lda, A,
@@ -535,12 +549,23 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(int r0, int rb, int nrh
     auto D = [&](int i, int k) { return sD[i + k * TRSM_NB]; };
     if(KIND == 0)
     {
+        // the block read through an offset tied to x[k] every two columns
+        // (an empty asm), so its entries are not all loaded ahead and spilled
+        const T* Dp = sD;
 #pragma unroll
         for(int k = 0; k < TRSM_NB; ++k)
+        {
+            if((k & 1) == 0)
+            {
+                uint32_t lo = 0;
+                asm volatile("" : "+v"(lo) : "v"(x[k]));
+                Dp = sD + lo;
+            }
 #pragma unroll
             for(int i = k + 1; i < TRSM_NB; ++i)
                 if(i < rb)
-                    x[i] = fma_t(-D(i, k), x[k], x[i]);
+                    x[i] = fma_t(-Dp[i + k * TRSM_NB], x[k], x[i]);
+        }
     }
     else if(KIND == 1)
     {
