@@ -54,6 +54,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
     constexpr int NW = PANEL_THREADS / 64;
     __shared__ T s_key[NW];
     __shared__ int s_idx[NW];
+    __shared__ T s_val[NW]; // each wavefront's candidate value
     __shared__ T s_pv;
     __shared__ int s_p;
     __shared__ int s_swap[PANEL_NB]; // row interchanged with row j0 + k (itself: none)
@@ -73,36 +74,45 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         const int c = j0 + k;
         if(PIVOT)
         {
-            T key = T(-1);
+            T key = T(-1), val = T(0);
             int idx = INT_MAX;
+#pragma unroll 4
             for(int i = c + tid; i < m; i += PANEL_THREADS)
             {
-                T kk = pivot_key(at(i, c), i == c);
+                const T a = at(i, c);
+                T kk = pivot_key(a, i == c);
                 if(kk > key)
                 {
                     key = kk;
                     idx = i;
+                    val = a;
                 }
             }
+            const int my_idx = idx;
             wave_argmax(key, idx);
             if(lane == 0)
             {
                 s_key[wid] = key;
                 s_idx[wid] = idx;
             }
+            // the lane holding the winning row publishes its value (rows are
+            // distinct across lanes), so no global read of it follows
+            if(my_idx == idx && idx != INT_MAX)
+                s_val[wid] = val;
             __syncthreads();
             if(tid == 0)
             {
                 T bk = s_key[0];
-                int bi = s_idx[0];
+                int bi = s_idx[0], bw = 0;
                 for(int w = 1; w < NW; ++w)
                     if(s_key[w] > bk || (s_key[w] == bk && s_idx[w] < bi))
                     {
                         bk = s_key[w];
                         bi = s_idx[w];
+                        bw = w;
                     }
                 s_p = bi;
-                s_pv = at(bi, c);
+                s_pv = s_val[bw];
                 ipivA[b * strideP + c] = bi + 1;
                 if(s_pv == T(0) && info[b] == 0)
                     info[b] = c + 1;
@@ -148,27 +158,41 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
             const int j = j0 + q;
             u[q] = (j > c && j < jend) ? at(c, j) : T(0);
         }
-        for(int i = c + 1 + tid; i < m; i += PANEL_THREADS)
+        // two rows per pass (i and i + PANEL_THREADS): both rows' loads are
+        // issued before either row's stores
+        for(int i = c + 1 + tid; i < m; i += 2 * PANEL_THREADS)
         {
-            T l = at(i, c);
-            T v[PANEL_NB];
+            const int i1 = i + PANEL_THREADS;
+            const bool has1 = i1 < m;
+            T l0 = at(i, c);
+            T l1 = has1 ? at(i1, c) : T(0);
+            T v0[PANEL_NB], v1[PANEL_NB];
 #pragma unroll
             for(int q = 0; q < PANEL_NB; ++q)
             {
                 const int j = j0 + q;
-                v[q] = (j > c && j < jend) ? at(i, j) : T(0);
+                const bool in = j > c && j < jend;
+                v0[q] = in ? at(i, j) : T(0);
+                v1[q] = (in && has1) ? at(i1, j) : T(0);
             }
             if(nz || !PIVOT)
             {
-                l *= rcp;
-                at(i, c) = l;
+                l0 *= rcp;
+                l1 *= rcp;
+                at(i, c) = l0;
+                if(has1)
+                    at(i1, c) = l1;
             }
 #pragma unroll
             for(int q = 0; q < PANEL_NB; ++q)
             {
                 const int j = j0 + q;
                 if(j > c && j < jend)
-                    at(i, j) = fma_t(-l, u[q], v[q]);
+                {
+                    at(i, j) = fma_t(-l0, u[q], v0[q]);
+                    if(has1)
+                        at(i1, j) = fma_t(-l1, u[q], v1[q]);
+                }
             }
         }
         __syncthreads();
