@@ -623,6 +623,66 @@ def measure_half_width(torch, ra, handle, A, B, reps: int):
     return out
 
 
+def measure_blocked(torch, ra, handle, reps: int):
+    """Blocked path (orders beyond the fused kernels, tall matrices): plain
+    FP64 dgetrf of 64 matrices of order 1024 and of 64 matrices of 4096 x 128,
+    and dgetrs of the square ones with 8 right-hand sides, beside the image's
+    rocSOLVER dgetrf on the same inputs (comparison only)."""
+    bc, nrhs = 64, 8
+    g = torch.Generator(device="cuda").manual_seed(20261205)
+    out = {"workload": "dgetrf n=1024 and 4096x128, batch 64 each; dgetrs n=1024, 8 RHS; N(0,1)"}
+    rs = None
+    try:
+        rs = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
+                                      "librocsolver.so"))
+        P, I, S = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        rs.rocsolver_dgetrf_strided_batched.argtypes = [P, I, I, P, I, S, P, S, P, I]
+    except OSError:
+        rs = None
+    for key, m, n in (("square_n1024", 1024, 1024), ("tall_4096x128", 4096, 128)):
+        A0 = torch.randn((bc, n, m), dtype=torch.float64, device="cuda", generator=g)
+        A = torch.empty_like(A0)
+        k = min(m, n)
+        ipiv = torch.zeros((bc, k), dtype=torch.int32, device="cuda")
+        info = torch.zeros(bc, dtype=torch.int32, device="cuda")
+
+        def getrf():
+            ra.check("rocsolver_dgetrf_strided_batched", handle.value, m, n, A.data_ptr(), m,
+                     m * n, ipiv.data_ptr(), k, info.data_ptr(), bc)
+
+        def ref():
+            rs.rocsolver_dgetrf_strided_batched(handle.value, m, n, A.data_ptr(), m, m * n,
+                                                ipiv.data_ptr(), k, info.data_ptr(), bc)
+        A.copy_(A0)
+        getrf()
+        t = event_ms(torch, getrf, reps, lambda: A.copy_(A0))
+        fl = (m * n * n - n ** 3 / 3.0 - n * n / 2.0) if m >= n else lu_flops(n)
+        r = {"dgetrf_ms": round(t, 4), "tflops": round(bc * fl / (t * 1e-3) / 1e12, 3),
+             "fp64_frac": round(bc * fl / (t * 1e-3) / 1e12 / 78.6, 4),
+             "nonzero_info": int((info != 0).sum())}
+        if key == "square_n1024":
+            B0 = torch.randn((bc, nrhs, n), dtype=torch.float64, device="cuda", generator=g)
+            B = torch.empty_like(B0)
+
+            def getrs():
+                ra.check("rocsolver_dgetrs_strided_batched", handle.value, ra.OP_NONE, n, nrhs,
+                         A.data_ptr(), n, n * n, ipiv.data_ptr(), n, B.data_ptr(), n, n * nrhs,
+                         bc)
+            A.copy_(A0)
+            getrf()
+            r["dgetrs_8rhs_ms"] = round(event_ms(torch, getrs, reps, lambda: B.copy_(B0)), 4)
+            del B0, B
+        if rs is not None:
+            A.copy_(A0)
+            ref()
+            tr = event_ms(torch, ref, reps, lambda: A.copy_(A0))
+            r["rocsolver_dgetrf_ms"] = round(tr, 4)
+            r["speedup_vs_rocsolver"] = round(tr / t, 2)
+        out[key] = r
+        del A0, A
+    return out
+
+
 def other_configs(args, torch, ra, A, B):
     """Every other configuration, measured in the same run (each guarded: a
     failure is reported, not hidden)."""
@@ -633,6 +693,7 @@ def other_configs(args, torch, ra, A, B):
                     ("c2_getrf_getrs_n128_b16384", lambda: measure_config2(torch, ra, handle, reps)),
                     ("c4_vbatched_gesv_10000", lambda: measure_config4(torch, ra, handle, reps)),
                     ("c5_adjoint_n96_b2048", lambda: measure_config5(torch, ra, handle, reps)),
+                    ("blocked_getrf_n1024_b64", lambda: measure_blocked(torch, ra, handle, reps)),
                     ("c3_half_width_factors",
                      lambda: measure_half_width(torch, ra, handle, A, B, reps))):
         try:
