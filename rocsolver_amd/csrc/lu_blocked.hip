@@ -55,8 +55,6 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
     __shared__ T s_key[NW];
     __shared__ int s_idx[NW];
     __shared__ T s_val[NW]; // each wavefront's candidate value
-    __shared__ T s_pv;
-    __shared__ int s_p;
     __shared__ int s_swap[PANEL_NB]; // row interchanged with row j0 + k (itself: none)
     __shared__ int s_pos[2 * PANEL_NB]; // rows touched by the panel's interchanges
     __shared__ int s_src[2 * PANEL_NB]; // original row whose entries end at s_pos[q]
@@ -72,6 +70,8 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
     for(int k = 0; k < jb; ++k)
     {
         const int c = j0 + k;
+        int p = c;
+        T pv;
         if(PIVOT)
         {
             T key = T(-1), val = T(0);
@@ -100,34 +100,32 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
             if(my_idx == idx && idx != INT_MAX)
                 s_val[wid] = val;
             __syncthreads();
+            // every thread combines the wavefronts' candidates itself (no
+            // serial section and no second barrier)
+            T bk = s_key[0];
+            int bw = 0;
+            p = s_idx[0];
+            for(int w = 1; w < NW; ++w)
+                if(s_key[w] > bk || (s_key[w] == bk && s_idx[w] < p))
+                {
+                    bk = s_key[w];
+                    p = s_idx[w];
+                    bw = w;
+                }
+            pv = s_val[bw];
             if(tid == 0)
             {
-                T bk = s_key[0];
-                int bi = s_idx[0], bw = 0;
-                for(int w = 1; w < NW; ++w)
-                    if(s_key[w] > bk || (s_key[w] == bk && s_idx[w] < bi))
-                    {
-                        bk = s_key[w];
-                        bi = s_idx[w];
-                        bw = w;
-                    }
-                s_p = bi;
-                s_pv = s_val[bw];
-                ipivA[b * strideP + c] = bi + 1;
-                if(s_pv == T(0) && info[b] == 0)
+                ipivA[b * strideP + c] = p + 1;
+                if(pv == T(0) && info[b] == 0)
                     info[b] = c + 1;
             }
         }
-        else if(tid == 0)
+        else
         {
-            s_p = c;
-            s_pv = at(c, c);
-            if(s_pv == T(0) && info[b] == 0)
+            pv = at(c, c);
+            if(tid == 0 && pv == T(0) && info[b] == 0)
                 info[b] = c + 1;
         }
-        __syncthreads();
-        const int p = s_p;
-        const T pv = s_pv;
         const bool nz = pv != T(0);
         if(tid == 0)
             s_swap[k] = (nz && p != c) ? p : c;
