@@ -23,6 +23,7 @@ namespace rsa
 
 constexpr int PANEL_NB = 32; // panel width
 constexpr int PANEL_THREADS = 256;
+constexpr int PANEL_TALL = 2048; // panels of this many rows or more: 2 * PANEL_THREADS
 constexpr int TRSM_NB = 64; // getrs diagonal block
 constexpr int64_t MAX_GRID_YZ = 65535; // grid y/z limit: larger batches are launched in chunks
 
@@ -44,14 +45,14 @@ rocblas_status reset_info(hipStream_t stream, int* info, int bc)
 }
 
 // ------------------------------------------------------------ panel + trsm
-template <typename T, bool PIVOT>
-__global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batch<T> AA, int lda,
+template <typename T, bool PIVOT, int NT>
+__global__ __launch_bounds__(NT) void panel_kernel(int m, int n, Batch<T> AA, int lda,
                                                               int* __restrict__ ipivA,
                                                               int64_t strideP,
                                                               int* __restrict__ info, int j0,
                                                               int jb)
 {
-    constexpr int NW = PANEL_THREADS / 64;
+    constexpr int NW = NT / 64;
     __shared__ T s_key[NW];
     __shared__ int s_idx[NW];
     __shared__ T s_val[NW]; // each wavefront's candidate value
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
             T key = T(-1), val = T(0);
             int idx = INT_MAX;
 #pragma unroll 4
-            for(int i = c + tid; i < m; i += PANEL_THREADS)
+            for(int i = c + tid; i < m; i += NT)
             {
                 const T a = at(i, c);
                 T kk = pivot_key(a, i == c);
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         const int jend = j0 + jb;
         if(nz && p != c)
         {
-            for(int j = j0 + tid; j < jend; j += PANEL_THREADS)
+            for(int j = j0 + tid; j < jend; j += NT)
             {
                 T t = at(c, j);
                 at(c, j) = at(p, j);
@@ -156,11 +157,11 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
             const int j = j0 + q;
             u[q] = (j > c && j < jend) ? at(c, j) : T(0);
         }
-        // two rows per pass (i and i + PANEL_THREADS): both rows' loads are
+        // two rows per pass (i and i + NT): both rows' loads are
         // issued before either row's stores
-        for(int i = c + 1 + tid; i < m; i += 2 * PANEL_THREADS)
+        for(int i = c + 1 + tid; i < m; i += 2 * NT)
         {
-            const int i1 = i + PANEL_THREADS;
+            const int i1 = i + NT;
             const bool has1 = i1 < m;
             T l0 = at(i, c);
             T l1 = has1 ? at(i1, c) : T(0);
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(PANEL_THREADS) void panel_kernel(int m, int n, Batc
         __syncthreads();
         const int cnt = s_cnt;
         const int nout = n - jb; // columns left and right of the panel
-        for(int e = tid; e < nout; e += PANEL_THREADS)
+        for(int e = tid; e < nout; e += NT)
         {
             const int j = e < j0 ? e : e + jb;
             T v[2 * PANEL_NB];
@@ -449,12 +450,24 @@ rocblas_status getrf_blocked(hipStream_t stream, int m, int n, Batch<T> A, int l
     for(int j = 0; j < kmax; j += PANEL_NB)
     {
         const int jb = (kmax - j) < PANEL_NB ? (kmax - j) : PANEL_NB;
-        if(pivot)
-            hipLaunchKernelGGL((panel_kernel<T, true>), dim3(bc), dim3(PANEL_THREADS), 0, stream,
-                               m, n, A, lda, ipiv, strideP, info, j, jb);
+        // tall panels (2048 rows or more) with 512 threads: half the rows
+        // per thread in each column step; shorter ones with 256, whose
+        // update keeps two rows' loads in flight without spilling
+        if(pivot && m >= PANEL_TALL)
+            hipLaunchKernelGGL((panel_kernel<T, true, 2 * PANEL_THREADS>), dim3(bc),
+                               dim3(2 * PANEL_THREADS), 0, stream, m, n, A, lda, ipiv, strideP,
+                               info, j, jb);
+        else if(pivot)
+            hipLaunchKernelGGL((panel_kernel<T, true, PANEL_THREADS>), dim3(bc), dim3(PANEL_THREADS),
+                               0, stream, m, n, A, lda, ipiv, strideP, info, j, jb);
+        else if(m >= PANEL_TALL)
+            hipLaunchKernelGGL((panel_kernel<T, false, 2 * PANEL_THREADS>), dim3(bc),
+                               dim3(2 * PANEL_THREADS), 0, stream, m, n, A, lda, ipiv, strideP,
+                               info, j, jb);
         else
-            hipLaunchKernelGGL((panel_kernel<T, false>), dim3(bc), dim3(PANEL_THREADS), 0, stream,
-                               m, n, A, lda, ipiv, strideP, info, j, jb);
+            hipLaunchKernelGGL((panel_kernel<T, false, PANEL_THREADS>), dim3(bc),
+                               dim3(PANEL_THREADS), 0, stream, m, n, A, lda, ipiv, strideP, info,
+                               j, jb);
         RSA_LAUNCH_CHECK();
         const int r0 = j + jb;
         if(r0 < n)

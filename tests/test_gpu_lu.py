@@ -58,7 +58,9 @@ def test_getrf_reference_matrices(torch_gpu, handle, m, lda, singular, n, dtype,
 @pytest.mark.parametrize("m,n", [(1, 1), (3, 7), (7, 3), (16, 16), (17, 17), (20, 31), (31, 20),
                                  (24, 32), (32, 9), (32, 32), (33, 33), (64, 64), (64, 8),
                                  (8, 64), (65, 65), (96, 96), (128, 128), (130, 70), (70, 130),
-                                 (256, 256), (300, 300), (512, 512), (600, 600)])
+                                 (256, 256), (300, 300), (512, 512), (600, 600),
+                                 # panels of 2048 rows or more: 512-thread panel kernel
+                                 (2100, 96), (2048, 300)])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_getrf_random(torch_gpu, handle, m, n, dtype):
     print(f"getrf random m={m} n={n} {dtype.__name__}", flush=True)
